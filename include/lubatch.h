@@ -1,0 +1,92 @@
+/*
+ * lubatch.h -- C ABI of the MI355X (gfx950) batched small-matrix LU inversion
+ * library (liblubatch.so).
+ *
+ * Every entry point takes plain pointers and sizes.  Matrix data is FP32,
+ * row-major, contiguous with stride n*n between matrices, and lives in device
+ * memory (HBM) owned by the caller.  Inversion is in place.  Functions return 0
+ * on success or a LUB_E* status; they never print and never exit the process.
+ *
+ * Reference interfaces replaced (sumukhashridhar/matrixInversion):
+ *   the kernel  template<T,matrixSize,threadsPerMatrix,matricesPerBlock,numMatrices>
+ *               __global__ void batched_lu_subwarp(T* A)
+ *               (parallel_pivot/luBatchedInplace.cuh:127-129,
+ *                serial_pivot/luBatchedInplace.cuh:104-106,
+ *                templated/luBatchedInplace.cuh:78-81)
+ *   its runtime-argument form
+ *               batched_lu_subwarp(T* A, int matrixSize, int numMatrices,
+ *                                  int threadsPerMatrix, int matricesPerBlock)
+ *               (no_templ/luBatchedInplace.cuh:82-85)
+ *   and the launch in each variant's main()
+ *               (e.g. parallel_pivot/luBatchedInplace.cu:118-133).
+ */
+#ifndef LUBATCH_H
+#define LUBATCH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define LUB_OK            0
+#define LUB_EINVAL_N      1   /* n outside [1, LUB_MAX_N]                     */
+#define LUB_EINVAL_BATCH  2   /* batch < 0                                    */
+#define LUB_ENULL         3   /* required pointer is NULL                     */
+#define LUB_EFLAGS        4   /* unknown flag bits or pivot rule              */
+#define LUB_EHIP          5   /* a HIP runtime call failed                    */
+#define LUB_ENODEVICE     6   /* no gfx950 device / library not initialised   */
+#define LUB_EALIGN        7   /* pointer not 4-byte aligned                   */
+#define LUB_EINVAL_ARG    8   /* other invalid argument (ragged tables, ...)  */
+
+#define LUB_MAX_N 32
+
+/* ---- flags ------------------------------------------------------------- */
+/* Pivot rule, bits 0..3.  The REF rules reproduce the reference: at step k the
+ * pivot is searched in column k of the row-permuted *input* matrix, which the
+ * Doolittle factorisation has not updated yet, so the pivot vector is a pure
+ * function of the input values. */
+#define LUB_PIVOT_NONE      0x0  /* no pivoting (templated/, no_templ/)                  */
+#define LUB_PIVOT_PARALLEL  0x1  /* REF rule, first maximum |a| wins ties, cross-lane
+                                    DPP argmax (parallel_pivot/ technique, serial_pivot/
+                                    tie rule); default                                   */
+#define LUB_PIVOT_SERIAL    0x2  /* REF rule, first-max, lane-serial scan (serial_pivot/) */
+#define LUB_PIVOT_TREE      0x3  /* REF rule with parallel_pivot/'s exact shared-memory
+                                    tree (luBatchedInplace.cuh:12-44): left slot wins
+                                    ties and, for n not a power of two, some lanes are
+                                    never reduced -- bit-compatible with that kernel     */
+#define LUB_PIVOT_MASK      0xF
+
+/* ---- entry points ------------------------------------------------------ */
+
+/* Invert `batch` n x n FP32 matrices at device pointer A in place.
+ * piv: device int32[batch*n] receiving perm[p] = original row at position p
+ *      (the reference's `pivots` array, never exported by the reference itself;
+ *      P A = L U with (P A)[p][:] = A[perm[p]][:]), or NULL.  With
+ *      LUB_PIVOT_NONE a non-NULL piv receives the identity.
+ * Synchronous: runs on the calling thread's default stream and waits. */
+int lu_batched_inverse(float* A, int* piv, int n, int batch, int flags);
+
+/* Same, enqueued on `stream` (a hipStream_t; NULL = default stream); returns
+ * after the launch without waiting.  A and piv must stay valid until the
+ * stream reaches the kernel. */
+int lu_batched_inverse_async(float* A, int* piv, int n, int batch, int flags, void* stream);
+
+/* 64-bit batch form of lu_batched_inverse_async. */
+int lu_batched_inverse_async64(float* A, int* piv, int n, int64_t batch, int flags, void* stream);
+
+/* Human-readable text of a status code (static storage). */
+const char* lu_status_string(int status);
+
+/* Text of the last HIP runtime failure seen by the calling thread ("" if none). */
+const char* lu_last_error(void);
+
+/* Library version, e.g. "0.1.0". */
+const char* lu_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUBATCH_H */

@@ -1,0 +1,82 @@
+"""ctypes binding of liblubatch.so (include/lubatch.h, include/lubatch_util.h).
+
+The library is built in-tree by ``python -m matrixinversion_amd.build`` into
+``matrixinversion_amd/_lib/liblubatch.so``.  There is no fallback: if the
+library is missing or cannot be loaded, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "liblubatch.so")
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes); the C-ABI of include/lubatch.h first.
+ABI = {
+    "lu_batched_inverse": (_i, [_vp, _vp, _i, _i, _i]),
+    "lu_batched_inverse_async": (_i, [_vp, _vp, _i, _i, _i, _vp]),
+    "lu_batched_inverse_async64": (_i, [_vp, _vp, _i, _i64, _i, _vp]),
+    "lu_status_string": (ctypes.c_char_p, [_i]),
+    "lu_last_error": (ctypes.c_char_p, []),
+    "lu_version": (ctypes.c_char_p, []),
+}
+
+UTIL = {
+    "lubx_device_count": (_i, []),
+    "lubx_device_info": (_i, [_i, ctypes.c_char_p, _i, ctypes.c_char_p, _i, ctypes.POINTER(_i),
+                              ctypes.POINTER(_i), ctypes.POINTER(_sz), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                              ctypes.POINTER(_sz)]),
+    "lubx_malloc": (_vp, [_sz]),
+    "lubx_free": (_i, [_vp]),
+    "lubx_host_alloc": (_vp, [_sz]),
+    "lubx_host_free": (_i, [_vp]),
+    "lubx_memcpy_htod": (_i, [_vp, _vp, _sz]),
+    "lubx_memcpy_dtoh": (_i, [_vp, _vp, _sz]),
+    "lubx_memcpy_dtod_async": (_i, [_vp, _vp, _sz, _vp]),
+    "lubx_memset": (_i, [_vp, _i, _sz]),
+    "lubx_device_sync": (_i, []),
+    "lubx_stream_create": (_vp, []),
+    "lubx_stream_destroy": (_i, [_vp]),
+    "lubx_stream_sync": (_i, [_vp]),
+    "lubx_event_create": (_vp, []),
+    "lubx_event_destroy": (_i, [_vp]),
+    "lubx_event_record": (_i, [_vp, _vp]),
+    "lubx_event_sync": (_i, [_vp]),
+    "lubx_event_elapsed_ms": (ctypes.c_float, [_vp, _vp]),
+    "lubx_copy_f32": (_i, [_vp, _vp, _sz, _vp]),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib() -> ctypes.CDLL:
+    """Load liblubatch.so once; raise if it is absent (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(f"{LIB_PATH} not found: build it with `python -m matrixinversion_amd.build`")
+        l = ctypes.CDLL(LIB_PATH)
+        for table in (ABI, UTIL):
+            for name, (res, args) in table.items():
+                fn = getattr(l, name)
+                fn.restype = res
+                fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        l = lib()
+        msg = l.lu_status_string(status).decode()
+        detail = l.lu_last_error().decode()
+        raise NativeError(f"{what or 'liblubatch'} failed: status {status} ({msg}) {detail}".strip())

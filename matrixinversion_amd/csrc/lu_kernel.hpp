@@ -1,0 +1,454 @@
+// lu_kernel.hpp -- gfx950 batched in-place inversion of small FP32 matrices.
+//
+// One wave64 owns a tile of 4 consecutive matrices; each matrix is owned by
+// one 16-lane DPP row of the wave, so every cross-lane broadcast a step needs
+// is a single `row_newbcast` DPP modifier and no LDS barrier is ever shared
+// between waves.  Per tile:
+//
+//   1. tile load     HBM -> LDS with coalesced dwordx4 loads (the 4 matrices
+//                    are contiguous: 16*n*n bytes)
+//   2. row read      lane l of matrix g holds original rows l and l+16
+//   3. pivot search  REF rule (parallel_pivot/luBatchedInplace.cuh:12-44 reads
+//                    column k of the row-permuted *input*; comp_U/comp_L at
+//                    :175-185 have not touched it yet), so the permutation is a
+//                    function of the input alone and is found before any
+//                    arithmetic:
+//                      PARALLEL  4-step DPP max of |a| bit patterns per column,
+//                                ties / NaN detected by ballot count and
+//                                resolved by the SERIAL scan for the wave
+//                      SERIAL    first-max scan over positions (strict >,
+//                                serial_pivot/luBatchedInplace.cuh:22-36)
+//                      TREE      parallel_pivot's shared-memory tree, exactly
+//   4. permute       rows written back to LDS at their pivot positions: P*A
+//   5. column read   lane q holds column q of P*A
+//   6. Gauss-Jordan  row operations on P*A, pivot-free (the permutation
+//                    already fixed the pivots, so the multipliers are those of
+//                    the reference's Doolittle LU of P*A), in registers, one
+//                    v_fmac_f32_dpp row_newbcast per update
+//   7. unpermute     column p of (P*A)^-1 is column perm[p] of A^-1 (the
+//                    reference's comp_inv folds P into the right-hand side the
+//                    same way, parallel_pivot/luBatchedInplace.cuh:99); it is
+//                    written as an LDS column with one runtime base per lane
+//   8. tile store    LDS -> HBM with coalesced dwordx4 stores, pivots as int32
+//
+// The LU-based inverse of the reference (Doolittle + two triangular solves)
+// and this Gauss-Jordan both perform no-pivot elimination on P*A with the
+// same pivots and multipliers; they differ in rounding order only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gj_asm.inc"
+
+namespace lub {
+
+constexpr int kWave = 64;
+constexpr int kLanesPerMatrix = 16;
+constexpr int kMatricesPerWave = kWave / kLanesPerMatrix;  // 4
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+enum Pivot : int { kPivNone = 0, kPivParallel = 1, kPivSerial = 2, kPivTree = 3 };
+
+// LDS geometry of one tile (in floats).  Rows whose length is a multiple of 8
+// floats get 4 floats of padding so that 8 consecutive rows start in 8
+// different 16-byte bank groups; matrices with n % 4 == 0 start 16 banks
+// apart so the two matrices of a half-wave column access do not collide.
+// For n % 4 != 0 the tile is stored contiguously (its dwordx4 chunks cross
+// row boundaries).
+template <int N>
+struct Geo {
+  static constexpr int R = (N + 15) / 16;  // rows (or columns) per lane
+  static constexpr int LD = (N % 8 == 0) ? N + 4 : N;
+  static constexpr int MS = (N % 4 == 0) ? ((N * LD + 31) / 32) * 32 + 16 : N * N;
+  static constexpr int TILE = kMatricesPerWave * MS;
+  static constexpr int PERM = TILE;                     // int32[4][32] after the tile
+  static constexpr int LDS_FLOATS = TILE + kMatricesPerWave * 32;
+};
+
+// ------------------------------------------------------------------ helpers
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xf, 0xf, true);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, true));
+}
+
+// max over the 16 lanes of a DPP row (quad xor 1, quad xor 2, half mirror, mirror).
+__device__ __forceinline__ uint32_t row_max_u32(uint32_t m) {
+  m = max(m, dpp_u32<0xB1>(m));   // quad_perm [1,0,3,2]
+  m = max(m, dpp_u32<0x4E>(m));   // quad_perm [2,3,0,1]
+  m = max(m, dpp_u32<0x141>(m));  // row_half_mirror
+  m = max(m, dpp_u32<0x140>(m));  // row_mirror
+  return m;
+}
+
+// Orders this wave's LDS accesses (the tile region is private to the wave;
+// the workgroup is one wave, so the barrier is free).
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+template <int N>
+__device__ __forceinline__ void lds_row_read(const float* p, float (&w)[N]) {
+  if constexpr (N % 4 == 0) {
+#pragma unroll
+    for (int j = 0; j < N; j += 4) {
+      f32x4 v = *reinterpret_cast<const f32x4*>(p + j);
+      w[j] = v.x; w[j + 1] = v.y; w[j + 2] = v.z; w[j + 3] = v.w;
+    }
+  } else if constexpr (N % 2 == 0) {
+#pragma unroll
+    for (int j = 0; j < N; j += 2) {
+      f32x2 v = *reinterpret_cast<const f32x2*>(p + j);
+      w[j] = v.x; w[j + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) w[j] = p[j];
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void lds_row_write(float* p, const float (&w)[N]) {
+  if constexpr (N % 4 == 0) {
+#pragma unroll
+    for (int j = 0; j < N; j += 4)
+      *reinterpret_cast<f32x4*>(p + j) = f32x4{w[j], w[j + 1], w[j + 2], w[j + 3]};
+  } else if constexpr (N % 2 == 0) {
+#pragma unroll
+    for (int j = 0; j < N; j += 2) *reinterpret_cast<f32x2*>(p + j) = f32x2{w[j], w[j + 1]};
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) p[j] = w[j];
+  }
+}
+
+// ------------------------------------------------------------- tile I/O
+
+// HBM -> LDS.  Full tiles of a 16-byte aligned batch move as dwordx4 chunks
+// (chunk c = floats 4c..4c+3 of the 4*n*n-float tile); the last, partial tile
+// and unaligned batches move element-wise, and matrices past the end of the
+// batch are filled with the identity so every lane computes on finite data.
+template <int N>
+__device__ __forceinline__ void tile_load(const float* __restrict__ src, float* lds, int valid, bool vec, int lane) {
+  using G = Geo<N>;
+  constexpr int NN = N * N;
+  if (vec && valid == kMatricesPerWave) {
+    constexpr int CH = NN;  // 4*n*n floats / 4
+    constexpr int IT = (CH + kWave - 1) / kWave;
+    f32x4 buf[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int c = lane + kWave * it;
+      if (CH % kWave == 0 || c < CH) buf[it] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + c);
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int c = lane + kWave * it;
+      if (CH % kWave == 0 || c < CH) {
+        const int f = 4 * c;
+        const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
+        *reinterpret_cast<f32x4*>(lds + m * G::MS + i * G::LD + j) = buf[it];
+      }
+    }
+  } else {
+    for (int f = lane; f < kMatricesPerWave * NN; f += kWave) {
+      const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
+      lds[m * G::MS + i * G::LD + j] = (m < valid) ? src[f] : (i == j ? 1.0f : 0.0f);
+    }
+  }
+}
+
+// LDS -> HBM, the mirror of tile_load; only the `valid` leading matrices are stored.
+template <int N>
+__device__ __forceinline__ void tile_store(float* __restrict__ dst, const float* lds, int valid, bool vec, int lane) {
+  using G = Geo<N>;
+  constexpr int NN = N * N;
+  if (vec && valid == kMatricesPerWave) {
+    constexpr int CH = NN;
+    constexpr int IT = (CH + kWave - 1) / kWave;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int c = lane + kWave * it;
+      if (CH % kWave == 0 || c < CH) {
+        const int f = 4 * c;
+        const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
+        f32x4 v = *reinterpret_cast<const f32x4*>(lds + m * G::MS + i * G::LD + j);
+        __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(dst) + c);
+      }
+    }
+  } else {
+    for (int f = lane; f < valid * NN; f += kWave) {
+      const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
+      dst[f] = lds[m * G::MS + i * G::LD + j];
+    }
+  }
+}
+
+// -------------------------------------------------------- pivot search
+
+// First-max scan over positions, identical in every lane of a matrix: the
+// rows still unplaced publish |a[.][k]| at their current position, then each
+// lane walks positions k..n-1 with strict '>' (serial_pivot/luBatchedInplace.cuh:
+// 25-34) and swaps (:134-139).  A NaN at position k is never beaten; a NaN
+// elsewhere never wins -- the reference's comparison semantics.
+template <int N>
+__device__ __forceinline__ void pivot_serial(const float (&w)[Geo<N>::R][N], int (&pos)[Geo<N>::R], float* col, int l) {
+  using G = Geo<N>;
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) pos[h] = l + 16 * h;
+  static_for<0, N>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+#pragma unroll
+    for (int h = 0; h < G::R; ++h)
+      if (l + 16 * h < N && pos[h] >= k) col[pos[h]] = fabsf(w[h][k]);
+    wave_sync();
+    int best = k;
+    float bv = col[k];
+#pragma unroll
+    for (int p = k + 1; p < N; ++p) {
+      const float v = col[p];
+      if (v > bv) { bv = v; best = p; }
+    }
+    wave_sync();
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) pos[h] = (pos[h] == k) ? best : ((pos[h] == best) ? k : pos[h]);
+  });
+}
+
+// parallel_pivot's find_pivot_parallel (luBatchedInplace.cuh:12-44) with
+// threadsPerMatrix = n, evaluated redundantly by every lane: slot t starts at
+// (|a_kk|, k) and takes position k+1+t when strictly larger; the tree then
+// walks stride = n/2, n/4, ..., 1 keeping the left slot unless it is strictly
+// smaller.  For n not a power of two some slots never reach slot 0, as there.
+template <int N>
+__device__ __forceinline__ void pivot_tree(const float (&w)[Geo<N>::R][N], int (&pos)[Geo<N>::R], float* col, int l) {
+  using G = Geo<N>;
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) pos[h] = l + 16 * h;
+  static_for<0, N>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+#pragma unroll
+    for (int h = 0; h < G::R; ++h)
+      if (l + 16 * h < N && pos[h] >= k) col[pos[h]] = fabsf(w[h][k]);
+    wave_sync();
+    const float vk = col[k];
+    float sv[N];
+    int si[N];
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+      sv[t] = vk;
+      si[t] = k;
+      if (k + 1 + t < N) {
+        const float v = col[k + 1 + t];
+        if (v > vk) { sv[t] = v; si[t] = k + 1 + t; }
+      }
+    }
+#pragma unroll
+    for (int stride = N / 2; stride > 0; stride >>= 1)
+#pragma unroll
+      for (int t = 0; t < stride; ++t)
+        if (sv[t] < sv[t + stride]) { sv[t] = sv[t + stride]; si[t] = si[t + stride]; }
+    const int best = si[0];
+    wave_sync();
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) pos[h] = (pos[h] == k) ? best : ((pos[h] == best) ? k : pos[h]);
+  });
+}
+
+// Cross-lane argmax: per column, each still-unplaced row offers the bit
+// pattern of |a| (monotone in |a| for non-NaN values), a 4-step DPP max finds
+// the largest, and the row(s) matching it are placed at position k.  With no
+// exact tie and no NaN the winner is unique and equals the first-max winner.
+// A tie is a step at which some matrix has more than one winning row: the
+// 16-bit group of the win ballot is then not a power of two (every group has
+// at least one winner while no earlier step tied, so no group field is zero
+// and the SWAR test x & (x - 0x0001000100010001) does not borrow across
+// groups).  Returns false after a tie or a NaN; the caller then reruns the
+// exact serial scan for the whole wave.
+template <int N>
+__device__ __forceinline__ bool pivot_parallel(const float (&w)[Geo<N>::R][N], int (&pos)[Geo<N>::R], int l) {
+  using G = Geo<N>;
+  bool rem[G::R];
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) { rem[h] = (l + 16 * h) < N; pos[h] = 0; }
+  uint64_t tie = 0;
+  bool bad = false;
+  static_for<0, N>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    uint32_t key[G::R];
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) key[h] = rem[h] ? (__builtin_bit_cast(uint32_t, w[h][k]) & 0x7fffffffu) : 0u;
+    uint32_t m = key[0];
+#pragma unroll
+    for (int h = 1; h < G::R; ++h) m = max(m, key[h]);
+    m = row_max_u32(m);
+    bad = bad || (m > 0x7f800000u);
+    uint64_t any = 0, both = ~uint64_t(0);
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) {
+      const bool win = rem[h] && key[h] == m;
+      const uint64_t b = __builtin_amdgcn_ballot_w64(win);
+      both = (h == 0) ? b : (both & b);
+      any |= b;
+      rem[h] = rem[h] && !win;
+      pos[h] = win ? k : pos[h];
+    }
+    if constexpr (G::R == 1) both = 0;
+    tie |= (any & (any - 0x0001000100010001ull)) | both;
+  });
+  return tie == 0 && __builtin_amdgcn_ballot_w64(bad) == 0;
+}
+
+// ------------------------------------------------------ Gauss-Jordan
+
+// In-place Gauss-Jordan with row operations on S = P*A held by columns: lane l
+// of the DPP row holds columns l and l+16 (w[h][i] = S[i][l + 16h]).  The
+// pivots are the diagonal of P*A in order, so the multipliers are those of
+// the reference's no-pivot Doolittle factorisation of P*A.  Step k:
+//   d = S[k][k] (broadcast from lane k%16), r = 1/d
+//   each column c != k:  S[i][c] -= S[i][k] * (S[k][c] * r)   for i != k,
+//     one v_fmac_f32_dpp per element: S[i][k] is broadcast from the lane
+//     owning column k (its entry k is zeroed first so row k is left alone)
+//     and t_c = -S[k][c] * r is the lane's own multiplier
+//   row k is normalised: S[k][c] *= r
+//   column k turns into right-hand column k.  It is kept unscaled: the right
+//     part R[i][k] = -S[i][k] / d is sigma_k * S[i][k] with sigma_k = -1/d,
+//     and every later row operation commutes with that column scale; its
+//     pivot entry becomes R[k][k] / sigma_k = -1.
+// At the end (P*A)^-1 = S * diag(sigma); sigma of the lane's columns is cs[].
+template <int N>
+__device__ __forceinline__ void gauss_jordan(float (&w)[Geo<N>::R][N], float (&cs)[Geo<N>::R], int l) {
+  using G = Geo<N>;
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) cs[h] = 1.0f;
+  static_for<0, N>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    constexpr int kl = k & 15, kh = k >> 4;
+    const float d = dpp_f32<0x150 + kl>(w[kh][k]);
+    float r = __builtin_amdgcn_rcpf(d);
+    r = __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);  // one Newton step
+    const bool pl = (l == kl);                              // this lane owns column k (half kh)
+    float t[G::R];
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) t[h] = -w[h][k] * r;
+    t[kh] = pl ? 0.0f : t[kh];
+    cs[kh] = pl ? -r : cs[kh];
+    w[kh][k] = pl ? 0.0f : w[kh][k];
+    if constexpr (G::R == 2) GjBlock<N, kl>::cross(w[1 - kh], w[kh], t[1 - kh]);
+    GjBlock<N, kl>::self(w[kh], t[kh]);
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) w[h][k] *= r;
+    w[kh][k] = pl ? -1.0f : w[kh][k];
+  });
+}
+
+// ------------------------------------------------------------ the kernel
+
+template <int N, int PIV>
+__device__ __forceinline__ void invert_tile(float* __restrict__ A, int* __restrict__ piv, int64_t m0, int valid,
+                                            bool vec, float* lds, int lane) {
+  using G = Geo<N>;
+  const int l = lane & 15;
+  const int g = lane >> 4;
+  float* const mat = lds + g * G::MS;
+  int* const perm = reinterpret_cast<int*>(lds + G::PERM) + g * 32;
+  float* const tile = A + m0 * (N * N);
+
+  tile_load<N>(tile, lds, valid, vec, lane);
+  wave_sync();
+
+  float w[G::R][N];
+  int prm[G::R];
+  if constexpr (PIV != kPivNone) {
+    // 2. original rows l and l+16 (rows past n read row n-1 and stay unused)
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) lds_row_read<N>(mat + min(l + 16 * h, N - 1) * G::LD, w[h]);
+    // 3. pivot positions of this lane's rows
+    int pos[G::R];
+    if constexpr (PIV == kPivParallel) {
+      if (!pivot_parallel<N>(w, pos, l)) {
+        wave_sync();
+        pivot_serial<N>(w, pos, mat, l);
+      }
+    } else if constexpr (PIV == kPivSerial) {
+      wave_sync();
+      pivot_serial<N>(w, pos, mat, l);
+    } else {
+      wave_sync();
+      pivot_tree<N>(w, pos, mat, l);
+    }
+    wave_sync();
+    // 4. P*A: row rho goes to position pos; perm[pos] = rho
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) {
+      if (l + 16 * h < N) {
+        lds_row_write<N>(mat + pos[h] * G::LD, w[h]);
+        perm[pos[h]] = l + 16 * h;
+      }
+    }
+    wave_sync();
+  }
+
+  // 5. column q of P*A -> lane q
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) {
+    const int q = min(l + 16 * h, N - 1);
+#pragma unroll
+    for (int p = 0; p < N; ++p) w[h][p] = mat[p * G::LD + q];
+    prm[h] = (PIV != kPivNone) ? (perm[q] & 31) : q;  // a permutation of 0..n-1; the mask keeps LDS indexing in range
+  }
+  wave_sync();
+
+  // 6. (P*A)^-1 = S * diag(cs), column q in lane q
+  float cs[G::R];
+  gauss_jordan<N>(w, cs, l);
+
+  // 7. column q of (P*A)^-1 is column perm[q] of A^-1
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) {
+    if (l + 16 * h < N) {
+      float* const colp = mat + prm[h];
+#pragma unroll
+      for (int i = 0; i < N; ++i) colp[i * G::LD] = w[h][i] * cs[h];
+    }
+  }
+  wave_sync();
+
+  // 8. store the inverses and the pivot vectors
+  tile_store<N>(tile, lds, valid, vec, lane);
+  if (piv != nullptr && g < valid) {
+#pragma unroll
+    for (int h = 0; h < G::R; ++h)
+      if (l + 16 * h < N) piv[(m0 + g) * N + l + 16 * h] = prm[h];
+  }
+  wave_sync();  // the next tile reuses the LDS region
+}
+
+template <int N, int PIV>
+__global__ __launch_bounds__(kWave) void lu_inverse_kernel(float* __restrict__ A, int* __restrict__ piv, int64_t batch,
+                                                           int64_t ntiles, int vec) {
+  __shared__ __attribute__((aligned(16))) float lds[Geo<N>::LDS_FLOATS];
+  const int lane = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t m0 = t * kMatricesPerWave;
+    const int64_t left = batch - m0;
+    const int valid = left < kMatricesPerWave ? (int)left : kMatricesPerWave;
+    invert_tile<N, PIV>(A, piv, m0, valid, vec != 0, lds, lane);
+  }
+}
+
+}  // namespace lub

@@ -1,0 +1,36 @@
+// lu_kernels_inst.hip -- instantiates lu_inverse_kernel<N, PIV> for
+// N in [LUB_N_LO, LUB_N_HI] and registers the host handles.  The build
+// compiles this file once per size range so the (large, fully unrolled)
+// kernels compile in parallel.
+#include "lu_kernel.hpp"
+#include "lub_internal.h"
+
+#ifndef LUB_N_LO
+#error "LUB_N_LO / LUB_N_HI must be defined"
+#endif
+
+#define LUB_CAT2(a, b, c) a##b##_##c
+#define LUB_CAT(a, b, c) LUB_CAT2(a, b, c)
+#define LUB_REGISTER_NAME LUB_CAT(lub_register_, LUB_N_LO, LUB_N_HI)
+
+namespace {
+
+template <int N>
+void register_size(lub::KernelTable* t) {
+  t->plain[N][lub::kPivNone] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, lub::kPivNone>);
+  t->plain[N][lub::kPivParallel] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, lub::kPivParallel>);
+  t->plain[N][lub::kPivSerial] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, lub::kPivSerial>);
+  t->plain[N][lub::kPivTree] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, lub::kPivTree>);
+}
+
+template <int N>
+void register_range(lub::KernelTable* t) {
+  if constexpr (N <= LUB_N_HI) {
+    register_size<N>(t);
+    register_range<N + 1>(t);
+  }
+}
+
+}  // namespace
+
+extern "C" void LUB_REGISTER_NAME(lub::KernelTable* t) { register_range<LUB_N_LO>(t); }
