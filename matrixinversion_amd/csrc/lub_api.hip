@@ -81,7 +81,7 @@ int lu_batched_inverse_async(float* A, int* piv, int n, int batch, int flags, vo
 
 int lu_batched_inverse(float* A, int* piv, int n, int batch, int flags) {
   int rc = lu_batched_inverse_async64(A, piv, n, static_cast<int64_t>(batch), flags, nullptr);
-  if (rc != LUB_OK) return rc;
+  if (rc != LUB_OK || batch == 0) return rc;
   hipError_t e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) {
     set_last_error("hipStreamSynchronize", e);

@@ -53,6 +53,8 @@ def _load():
         lib.orc_mt19937_uniform.argtypes = [ctypes.c_uint32, _c_float_p, ctypes.c_size_t]
         lib.orc_pivots.argtypes = [_c_float_p, ctypes.c_int, ctypes.c_int, _c_int_p]
         lib.orc_pivots.restype = ctypes.c_int
+        lib.orc_lu_factor_f32.argtypes = [_c_float_p, _c_float_p, ctypes.c_int, ctypes.c_int, _c_int_p]
+        lib.orc_lu_factor_f32.restype = ctypes.c_int
         lib.orc_lu_inverse_f32.argtypes = [_c_float_p, ctypes.c_int, ctypes.c_int, _c_int_p]
         lib.orc_lu_inverse_f32.restype = ctypes.c_int
         lib.orc_batch_inverse_f32.argtypes = [_c_float_p, _c_int_p, ctypes.c_int, ctypes.c_longlong,
@@ -133,6 +135,19 @@ def lu_inverse_f32(a: np.ndarray, rule: int = PIV_FIRSTMAX, threads: int = 0):
     if rc != 0:
         raise ValueError("orc_batch_inverse_f32 failed")
     return x, piv
+
+
+def lu_factor_f32(a: np.ndarray, rule: int = PIV_FIRSTMAX):
+    """Doolittle LU of P*A for one matrix in the reference's order.
+    Returns (LU, perm) with unit-lower L below the diagonal and U on/above."""
+    n = a.shape[-1]
+    a, ap = _f32(a)
+    lu = np.empty((n, n), dtype=np.float32)
+    perm = np.empty(n, dtype=np.int32)
+    rc = _load().orc_lu_factor_f32(ap, lu.ctypes.data_as(_c_float_p), n, rule, perm.ctypes.data_as(_c_int_p))
+    if rc != 0:
+        raise ValueError("orc_lu_factor_f32 failed")
+    return lu, perm
 
 
 def openmp_threads() -> int:

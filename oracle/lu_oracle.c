@@ -17,9 +17,10 @@
  *                         strides n/2..1, left slot kept unless strictly less)
  *                         and the swap at :161-172, with threadsPerMatrix = n
  *                         (parallel_pivot/luBatchedInplace.cu:8).
- *   orc_lu_inverse_f32    Doolittle comp_U / comp_L (parallel_pivot/
- *                         luBatchedInplace.cuh:60-82, the same in every variant),
- *                         then comp_inv (:84-125): column c solves L y = P e_c
+ *   orc_lu_factor_f32     Doolittle comp_U / comp_L (parallel_pivot/
+ *                         luBatchedInplace.cuh:60-82, the same in every variant)
+ *   orc_lu_inverse_f32    the factorisation, then comp_inv
+ *                         (:84-125): column c solves L y = P e_c
  *                         with (P e_c)_i = [perm[i] == c], then U x = y; the
  *                         no-pivot RHS is e_c (templated/luBatchedInplace.cuh:56).
  *   orc_verify_inv        verifyInv, parallel_pivot/verify.hpp:50-103: an
@@ -161,18 +162,17 @@ int orc_pivots(const float* A, int n, int rule, int* perm) {
 
 /* ------------------------------------------------------- FP32 inversion */
 
-/* In-place FP32 inverse of one n x n row-major matrix following the reference's
- * arithmetic order: row swaps by `rule`, Doolittle comp_U then comp_L for each
- * k (parallel_pivot/luBatchedInplace.cuh:60-82, 156-186), then one comp_inv per
- * column (:84-125) whose results are written back only after every column has
- * been solved (the reference's lanes finish their solves before the store loop).
- * perm (may be NULL) receives the pivot vector.  Returns 0. */
-int orc_lu_inverse_f32(float* A, int n, int rule, int* perm_out) {
+/* FP32 Doolittle factorisation of P*A in the reference's order: rows swapped
+ * by `rule`, then for each k comp_U (row k of U, parallel_pivot/
+ * luBatchedInplace.cuh:60-70) and comp_L (column k of L, :72-82); LU receives
+ * L (unit diagonal implied) below and U on/above the diagonal, as the
+ * reference leaves them in shared memory before comp_inv.  perm (may be NULL)
+ * receives the pivot vector.  Returns 0. */
+int orc_lu_factor_f32(const float* A, float* LU, int n, int rule, int* perm_out) {
     int perm[ORC_MAXN];
-    float M[ORC_MAXN * ORC_MAXN];
-    float X[ORC_MAXN * ORC_MAXN];
     if (n < 1 || n > ORC_MAXN) return -1;
     if (orc_pivots(A, n, rule, perm) != 0) return -1;
+    float* M = LU;
     for (int p = 0; p < n; ++p)
         memcpy(&M[p * n], &A[(size_t)perm[p] * n], sizeof(float) * (size_t)n);
     for (int k = 0; k < n; ++k) {
@@ -187,6 +187,21 @@ int orc_lu_inverse_f32(float* A, int n, int rule, int* perm_out) {
             M[i * n + k] = (M[i * n + k] - sum) / M[k * n + k];
         }
     }
+    if (perm_out) memcpy(perm_out, perm, sizeof(int) * (size_t)n);
+    return 0;
+}
+
+/* In-place FP32 inverse of one n x n row-major matrix following the reference's
+ * arithmetic order: orc_lu_factor_f32, then one comp_inv per column
+ * (parallel_pivot/luBatchedInplace.cuh:84-125) whose results are written back
+ * only after every column has been solved (the reference's lanes finish their
+ * solves before the store loop).  perm (may be NULL) receives the pivot
+ * vector.  Returns 0. */
+int orc_lu_inverse_f32(float* A, int n, int rule, int* perm_out) {
+    int perm[ORC_MAXN];
+    float M[ORC_MAXN * ORC_MAXN];
+    float X[ORC_MAXN * ORC_MAXN];
+    if (orc_lu_factor_f32(A, M, n, rule, perm) != 0) return -1;
     for (int c = 0; c < n; ++c) { /* comp_inv for column c */
         float y[ORC_MAXN], x[ORC_MAXN];
         for (int i = 0; i < n; ++i) {

@@ -1,0 +1,291 @@
+"""Pins the CPU oracle (CPU only).
+
+Each check is chosen so that a plausible mistake in the oracle -- a swapped
+index, a different tie rule, a dropped term -- fails it:
+  * the reference's own host checker (verify.hpp: pivotedA, verifyInv,
+    verifyLUwithPivoting, calc_cond_num) through the committed golden fixtures
+    in tests/golden/ (made by tests/golden/make_golden.py), and live when
+    oracle/_ref was built in this container;
+  * exact rational inverses of the reference driver's hand matrices;
+  * numbers published with the reference analysis (kappa_1 of the fixture
+    prefixes, where the tree rule departs from first-max, MT19937 outputs);
+  * independent libraries: numpy's MT19937 stream and LAPACK inverses;
+  * hand-derived outcomes of parallel_pivot's tree reduction.
+"""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import GOLDEN
+
+
+def _fixture(name):
+    return np.loadtxt(os.path.join(GOLDEN, name), dtype=np.float32).ravel()
+
+
+def _json(name):
+    with open(os.path.join(GOLDEN, name)) as fh:
+        return json.load(fh)
+
+
+# ----------------------------------------------------------------- MT19937
+
+def test_mt19937_published_outputs():
+    assert oracle.mt19937_raw(42, 3).tolist() == [1608637542, 3421126067, 4083286876]
+
+
+def test_mt19937_matches_numpy_legacy_seeding():
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(12345)
+    assert (oracle.mt19937_raw(12345, 5000) == bg.random_raw(5000).astype(np.uint32)).all()
+
+
+def test_uniform_mapping_is_exact():
+    raw = oracle.mt19937_raw(42, 4096)
+    a = oracle.uniform_matrices(42, 4, 32).ravel()
+    # a = 2*(x>>8)/2^24 - 1, exact: a multiple of 2^-23 in [-1, 1)
+    assert (a.astype(np.float64) == (raw >> 8).astype(np.float64) / 2**23 - 1.0).all()
+    assert a.min() >= -1.0 and a.max() < 1.0
+
+
+# ------------------------------------------------------------------ pivots
+
+@pytest.mark.parametrize("fixture", ["mtrand32_new1.txt", "mtrand32_new.txt", "mtrand32.txt"])
+def test_firstmax_pivots_equal_reference_pivotedA(fixture):
+    """pivotedA (parallel_pivot/verify.hpp:106-155) on every n x n prefix the
+    drivers read; mtrand32_new.txt has integer entries, so exact ties."""
+    golden = _json("pivots_pivotedA.json")["fixture_prefixes"][fixture]
+    t = _fixture(fixture)
+    for n in range(1, 33):
+        a = t[: n * n].reshape(n, n)
+        assert oracle.pivots(a, oracle.PIV_FIRSTMAX).tolist() == golden[str(n)], f"n={n}"
+
+
+def test_firstmax_pivots_random_cases():
+    for case in _json("pivots_pivotedA.json")["random"]:
+        n = case["n"]
+        a = np.array(case["a"], dtype=np.float32).reshape(n, n)
+        assert oracle.pivots(a, oracle.PIV_FIRSTMAX).tolist() == case["pivots"]
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built from the reference sources")
+def test_firstmax_pivots_live_reference():
+    rng = np.random.RandomState(3)
+    for _ in range(200):
+        n = int(rng.randint(1, 33))
+        a = rng.randint(-3, 4, size=(n, n)).astype(np.float32) if rng.rand() < 0.5 else \
+            rng.uniform(-1, 1, size=(n, n)).astype(np.float32)
+        ref, _ = oracle.ref_pivotedA(a)
+        assert (oracle.pivots(a, oracle.PIV_FIRSTMAX) == ref).all()
+
+
+def test_tree_hand_derived_dropped_slot():
+    """n = 12: strides 6, 3, 1 never merge slot 2 (rows 3, 6, 9, 12 at step 0 are
+    lanes 2, 5, 8, 11 -> slots {2, 5, 8, 11}) into slot 0.  Put the column maximum
+    in row 3: first-max picks row 3, the tree picks the largest surviving row."""
+    n = 12
+    a = np.eye(n, dtype=np.float32)
+    a[:, 0] = [0.1, 0.2, 0.3, 0.9, 0.1, 0.1, 0.1, 0.5, 0.1, 0.1, 0.1, 0.1]
+    assert oracle.pivots(a, oracle.PIV_FIRSTMAX)[0] == 3
+    assert oracle.pivots(a, oracle.PIV_TREE)[0] == 7
+
+
+def test_tree_hand_derived_tie():
+    """n = 4, column 0 = [0, 1, 3, 3]: slots (1,r1), (3,r2), (3,r3), (0,r0);
+    stride 2 gives slot0 = (3,r3), slot1 = (3,r2); stride 1 keeps slot 0 on the
+    tie, so the tree picks row 3 while first-max picks row 2."""
+    a = np.eye(4, dtype=np.float32)
+    a[:, 0] = [0, 1, 3, 3]
+    assert oracle.pivots(a, oracle.PIV_FIRSTMAX)[0] == 2
+    assert oracle.pivots(a, oracle.PIV_TREE)[0] == 3
+
+
+def test_tree_vs_firstmax_on_fixture_prefixes():
+    """Published with the reference analysis: on mtrand32_new1.txt prefixes the
+    tree rule departs from first-max exactly for n in {12, 20, 24, 28} (of
+    4..32 step 4), for n = 12 first at step k = 8; and on the integer fixture
+    mtrand32_new.txt it departs for n = 16 and 32 because of ties."""
+    t = _fixture("mtrand32_new1.txt")
+    differ = []
+    for n in range(4, 33, 4):
+        a = t[: n * n].reshape(n, n)
+        fm, tr = oracle.pivots(a, oracle.PIV_FIRSTMAX), oracle.pivots(a, oracle.PIV_TREE)
+        if (fm != tr).any():
+            differ.append(n)
+    assert differ == [12, 20, 24, 28]
+    a = t[:144].reshape(12, 12)
+    fm, tr = oracle.pivots(a, oracle.PIV_FIRSTMAX), oracle.pivots(a, oracle.PIV_TREE)
+    assert int(np.nonzero(fm != tr)[0][0]) == 8
+    ti = _fixture("mtrand32_new.txt")
+    for n in (16, 32):
+        a = ti[: n * n].reshape(n, n)
+        assert (oracle.pivots(a, oracle.PIV_FIRSTMAX) != oracle.pivots(a, oracle.PIV_TREE)).any()
+
+
+def test_tree_equals_firstmax_pow2_without_ties():
+    """For a power-of-two n the tree visits every slot; with distinct values it
+    must find the unique maximum, i.e. agree with first-max."""
+    rng = np.random.RandomState(5)
+    for n in (2, 4, 8, 16, 32):
+        for _ in range(50):
+            a = rng.uniform(-1, 1, size=(n, n)).astype(np.float32)
+            assert (oracle.pivots(a, oracle.PIV_TREE) == oracle.pivots(a, oracle.PIV_FIRSTMAX)).all()
+
+
+def test_seed42_batch_tree_equals_firstmax():
+    """Published: seed-42 U[-1,1] n=32 x 1000 has no exact ties, so the two
+    rules agree on all 1000."""
+    a = oracle.uniform_matrices(42, 1000, 32)
+    assert (oracle.batch_pivots(a, oracle.PIV_TREE) == oracle.batch_pivots(a, oracle.PIV_FIRSTMAX)).all()
+
+
+def test_ref_rule_differs_from_gepp():
+    """The reference searches the un-updated column, not LAPACK's updated one:
+    on the fixture they agree at n=4 only, and on random U[-1,1] n=32 matrices
+    they (almost) never agree."""
+    t = _fixture("mtrand32_new1.txt")
+    same = []
+    for n in range(4, 33, 4):
+        a = t[: n * n].reshape(n, n)
+        _, gp, _ = oracle.inverse_f64(a)
+        same.append(bool((gp == oracle.pivots(a)).all()))
+    assert same == [True] + [False] * 7
+    a = oracle.uniform_matrices(9, 200, 32)
+    agree = sum(bool((oracle.inverse_f64(m)[1] == oracle.pivots(m)).all()) for m in a)
+    assert agree == 0
+
+
+def test_pivot_nan_and_zero_semantics():
+    """Strict '>' semantics: a NaN at position k keeps k, a NaN elsewhere never
+    wins, an all-zero column keeps k."""
+    a = np.eye(4, dtype=np.float32)
+    a[:, 0] = [np.nan, 5, 1, 2]
+    assert oracle.pivots(a)[0] == 0
+    a[:, 0] = [1, np.nan, 5, 2]
+    assert oracle.pivots(a)[0] == 2
+    a[:, 0] = [0, 0, 0, 0]
+    assert oracle.pivots(a)[0] == 0
+
+
+# ---------------------------------------------------------------- inverses
+
+def test_kat_exact_inverses():
+    """Hand matrices of parallel_pivot/luBatchedInplace.cu:20-25 against their
+    exact rational inverses, under every pivot rule."""
+    for name, case in _json("kat_exact_inverses.json").items():
+        n = case["n"]
+        a = np.array(case["a"], dtype=np.float32).reshape(n, n)
+        exact = np.array([[float(Fraction(v)) for v in row] for row in case["inverse"]])
+        for rule in (oracle.PIV_FIRSTMAX, oracle.PIV_TREE, oracle.PIV_NONE):
+            x, _ = oracle.lu_inverse_f32(a[None], rule)
+            err = np.abs(x[0] - exact).max() / np.abs(exact).max()
+            assert err < 64 * n * 2**-24 * oracle.cond1(a), f"{name} rule={rule} err={err:.3e}"
+        x64, _, info = oracle.inverse_f64(a)
+        assert info == 0 and np.abs(x64 - exact).max() < 1e-12 * np.abs(exact).max() * oracle.cond1(a)
+
+
+def test_f64_inverse_against_lapack():
+    rng = np.random.RandomState(1)
+    for n in (1, 2, 5, 17, 32):
+        a = rng.uniform(-1, 1, size=(n, n)).astype(np.float32)
+        x64, _, info = oracle.inverse_f64(a)
+        ref = np.linalg.inv(a.astype(np.float64))
+        assert info == 0
+        assert np.abs(x64 - ref).max() <= 1e-13 * np.linalg.cond(a.astype(np.float64), 1) * np.abs(ref).max()
+
+
+def test_f64_gepp_pivots_against_lapack_getrf():
+    import scipy.linalg as sla
+
+    rng = np.random.RandomState(2)
+    for n in (3, 8, 32):
+        a = rng.uniform(-1, 1, size=(n, n)).astype(np.float32)
+        _, perm, _ = oracle.inverse_f64(a)
+        p, _, _ = sla.lu(a.astype(np.float64))  # a = p @ l @ u, so rows of a in order perm: p.T @ a
+        assert (np.argmax(p, axis=0) == perm).all()
+
+
+def test_brute_force_adjugate_small():
+    """n <= 3: the FP32 restatement against the adjugate formula in FP64."""
+    rng = np.random.RandomState(4)
+    for n in (1, 2, 3):
+        for _ in range(50):
+            a = rng.uniform(-1, 1, size=(n, n)).astype(np.float32)
+            a64 = a.astype(np.float64)
+            if n == 1:
+                adj = np.array([[1.0]])
+            elif n == 2:
+                adj = np.array([[a64[1, 1], -a64[0, 1]], [-a64[1, 0], a64[0, 0]]])
+            else:
+                adj = np.empty((3, 3))
+                for i in range(3):
+                    for j in range(3):
+                        m = np.delete(np.delete(a64, j, 0), i, 1)
+                        adj[i, j] = (-1) ** (i + j) * (m[0, 0] * m[1, 1] - m[0, 1] * m[1, 0])
+            exact = adj / np.linalg.det(a64)
+            x, _ = oracle.lu_inverse_f32(a[None], oracle.PIV_FIRSTMAX)
+            k = oracle.cond1(a)
+            assert np.abs(x[0] - exact).max() <= 64 * n * k * 2**-24 * np.abs(exact).max()
+
+
+def test_doolittle_factors_pass_reference_lu_check():
+    """verifyLUwithPivoting (parallel_pivot/verify.hpp:157-242) accepted the
+    oracle's factors of P*A (golden); and L U reproduces P A."""
+    for case in _json("verifyLU_counts.json"):
+        assert case["correct"] == 1
+        t = _fixture(case["fixture"])
+        n = case["n"]
+        a = t[: n * n].reshape(n, n)
+        lu, perm = oracle.lu_factor_f32(a)
+        L = np.tril(lu.astype(np.float64), -1) + np.eye(n)
+        U = np.triu(lu.astype(np.float64))
+        assert np.abs(L @ U - a[perm]).max() < 1e-3
+
+
+def test_no_pivot_rhs_is_identity_column():
+    """templated/luBatchedInplace.cuh:56: with no pivoting the right-hand side is
+    e_c, so the pivot vector is the identity and the result is A^-1."""
+    a = oracle.dominant_matrices(3, 8, 9)
+    x, piv = oracle.lu_inverse_f32(a, oracle.PIV_NONE)
+    assert (piv == np.arange(9)).all()
+    ref = np.linalg.inv(a.astype(np.float64))
+    assert np.abs(x - ref).max() < 1e-5
+
+
+# ----------------------------------------------------------------- checks
+
+def test_verify_inv_counts_match_reference():
+    for case in _json("verifyInv_counts.json"):
+        n = case["n"]
+        a = np.array(case["a"], dtype=np.float32).reshape(-1, n, n)
+        x = np.array(case["x"], dtype=np.float32).reshape(-1, n, n)
+        correct, _ = oracle.verify_inv(a, x)
+        assert correct == case["correct"]
+        assert case["correct"] == 2 and case["wrong"] == 4
+
+
+def test_cond1_against_published_and_lapack():
+    """kappa_1 of the mtrand32_new1.txt prefixes as published with the reference
+    analysis, and numpy.linalg.cond(., 1)."""
+    published = {4: 26.3, 8: 229.2, 12: 152.6, 16: 604.5, 20: 1428.8, 24: 697.3, 28: 8512.4, 32: 377.6}
+    t = _fixture("mtrand32_new1.txt")
+    for n, k in published.items():
+        a = t[: n * n].reshape(n, n)
+        assert round(oracle.cond1(a), 1) == k
+        assert abs(oracle.cond1(a) / np.linalg.cond(a.astype(np.float64), 1) - 1) < 1e-9
+
+
+def test_reference_calc_cond_num_returns_norm():
+    """The reference's calc_cond_num (verify.hpp:245-338) eliminates without an
+    identity augment, so its 'inverse' is the reduced echelon form (I) and the
+    value it returns is ||A||_1 -- recorded, not reproduced, by the oracle."""
+    golden = _json("calc_cond_num.json")["mtrand32_new1.txt"]
+    t = _fixture("mtrand32_new1.txt")
+    for n in range(1, 33):
+        a = t[: n * n].reshape(n, n)
+        norm1 = np.abs(a.astype(np.float64)).sum(axis=0).max()
+        assert abs(golden[str(n)] / norm1 - 1) < 1e-4, n
