@@ -97,6 +97,29 @@ __device__ __forceinline__ uint32_t row_max_u32(uint32_t m) {
   return m;
 }
 
+// key = live ? bits(|x|) : 0 in one v_cndmask with the |x| source modifier.
+// Written as one instruction so that |x| is not computed ahead for every
+// column (it depends on the lane mask of still-unplaced rows of this step).
+__device__ __forceinline__ uint32_t abs_key(float x, bool live) {
+  uint32_t k;
+  asm("v_cndmask_b32_e64 %0, 0, |%1|, %2" : "=v"(k) : "v"(x), "s"(__builtin_amdgcn_ballot_w64(live)));
+  return k;
+}
+
+// Materialises v in a VGPR at this point of the program: keeps a running
+// reduction from being re-associated into a tree over all its inputs (which
+// would keep every step's value live until the end).
+__device__ __forceinline__ void keep_in_vgpr(uint32_t& v) { asm volatile("" : "+v"(v)); }
+
+// bitwise OR over the 16 lanes of a DPP row.
+__device__ __forceinline__ uint32_t row_or_u32(uint32_t m) {
+  m |= dpp_u32<0xB1>(m);
+  m |= dpp_u32<0x4E>(m);
+  m |= dpp_u32<0x141>(m);
+  m |= dpp_u32<0x140>(m);
+  return m;
+}
+
 // Orders this wave's LDS accesses (the tile region is private to the wave;
 // the workgroup is one wave, so the barrier is free).
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
@@ -139,54 +162,96 @@ __device__ __forceinline__ void lds_row_write(float* p, const float (&w)[N]) {
 // ------------------------------------------------------------- tile I/O
 
 // HBM -> LDS.  Full tiles of a 16-byte aligned batch move as dwordx4 chunks
-// (chunk c = floats 4c..4c+3 of the 4*n*n-float tile); the last, partial tile
-// and unaligned batches move element-wise, and matrices past the end of the
-// batch are filled with the identity so every lane computes on finite data.
+// (chunk c = floats 4c..4c+3 of the 4*n*n-float tile), fetched into registers
+// one tile ahead (TileRegs) and staged into LDS when the tile's turn comes.
+// The last, partial tile and unaligned batches move element-wise, and matrices
+// past the end of the batch are filled with the identity so every lane
+// computes on finite data.
 template <int N>
-__device__ __forceinline__ void tile_load(const float* __restrict__ src, float* lds, int valid, bool vec, int lane) {
+struct TileRegs {
+  static constexpr int CH = N * N;  // dwordx4 chunks per tile (4*n*n floats / 4)
+  static constexpr int IT = (CH + kWave - 1) / kWave;
+  f32x4 v[IT];
+};
+
+// LDS offset (floats) of dwordx4 chunk c = lane + 64*it of the tile.  When n
+// divides 256 (one wave iteration covers whole rows) it splits into a per-lane
+// base and a compile-time offset per iteration, so the staging loop needs one
+// address register instead of one per chunk.
+template <int N>
+__device__ __forceinline__ int chunk_lds_base(int lane) {
   using G = Geo<N>;
   constexpr int NN = N * N;
-  if (vec && valid == kMatricesPerWave) {
-    constexpr int CH = NN;  // 4*n*n floats / 4
-    constexpr int IT = (CH + kWave - 1) / kWave;
-    f32x4 buf[IT];
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int c = lane + kWave * it;
-      if (CH % kWave == 0 || c < CH) buf[it] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + c);
-    }
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int c = lane + kWave * it;
-      if (CH % kWave == 0 || c < CH) {
-        const int f = 4 * c;
-        const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
-        *reinterpret_cast<f32x4*>(lds + m * G::MS + i * G::LD + j) = buf[it];
-      }
-    }
+  if constexpr (256 % N == 0 && N >= 8) {
+    const int f = 4 * lane;                        // within the first iteration
+    const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
+    return m * G::MS + i * G::LD + j;
   } else {
-    for (int f = lane; f < kMatricesPerWave * NN; f += kWave) {
-      const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
-      lds[m * G::MS + i * G::LD + j] = (m < valid) ? src[f] : (i == j ? 1.0f : 0.0f);
-    }
+    return 0;
   }
 }
 
-// LDS -> HBM, the mirror of tile_load; only the `valid` leading matrices are stored.
+template <int N>
+__device__ __forceinline__ int chunk_lds_offset(int base, int lane, int it) {
+  using G = Geo<N>;
+  constexpr int NN = N * N;
+  if constexpr (256 % N == 0 && N >= 8) {
+    // iteration it adds 256 floats: 256/N rows, and whole matrices past NN
+    const int f0 = 256 * it;
+    const int m = f0 / NN, e = f0 - m * NN, i = e / N;
+    return base + m * G::MS + i * G::LD;            // it is a compile-time constant after unrolling
+  } else {
+    const int f = 4 * (lane + kWave * it);
+    const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
+    return m * G::MS + i * G::LD + j;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void tile_fetch(const float* __restrict__ src, TileRegs<N>& r, int lane) {
+  using T = TileRegs<N>;
+#pragma unroll
+  for (int it = 0; it < T::IT; ++it) {
+    const int c = lane + kWave * it;
+    if (T::CH % kWave == 0 || c < T::CH) r.v[it] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + c);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void tile_stage(const TileRegs<N>& r, float* lds, int lane) {
+  using T = TileRegs<N>;
+  const int base = chunk_lds_base<N>(lane);
+#pragma unroll
+  for (int it = 0; it < T::IT; ++it) {
+    const int c = lane + kWave * it;
+    if (T::CH % kWave == 0 || c < T::CH)
+      *reinterpret_cast<f32x4*>(lds + chunk_lds_offset<N>(base, lane, it)) = r.v[it];
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void tile_load_elementwise(const float* __restrict__ src, float* lds, int valid, int lane) {
+  using G = Geo<N>;
+  constexpr int NN = N * N;
+  for (int f = lane; f < kMatricesPerWave * NN; f += kWave) {
+    const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
+    lds[m * G::MS + i * G::LD + j] = (m < valid) ? src[f] : (i == j ? 1.0f : 0.0f);
+  }
+}
+
+// LDS -> HBM, the mirror of the load; only the `valid` leading matrices are stored.
 template <int N>
 __device__ __forceinline__ void tile_store(float* __restrict__ dst, const float* lds, int valid, bool vec, int lane) {
   using G = Geo<N>;
+  using T = TileRegs<N>;
   constexpr int NN = N * N;
   if (vec && valid == kMatricesPerWave) {
-    constexpr int CH = NN;
-    constexpr int IT = (CH + kWave - 1) / kWave;
+    const int base = chunk_lds_base<N>(lane);
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
+    for (int it = 0; it < T::IT; ++it) {
       const int c = lane + kWave * it;
-      if (CH % kWave == 0 || c < CH) {
-        const int f = 4 * c;
-        const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
-        f32x4 v = *reinterpret_cast<const f32x4*>(lds + m * G::MS + i * G::LD + j);
+      if (T::CH % kWave == 0 || c < T::CH) {
+        f32x4 v = *reinterpret_cast<const f32x4*>(lds + chunk_lds_offset<N>(base, lane, it));
         __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(dst) + c);
       }
     }
@@ -273,44 +338,45 @@ __device__ __forceinline__ void pivot_tree(const float (&w)[Geo<N>::R][N], int (
 // pattern of |a| (monotone in |a| for non-NaN values), a 4-step DPP max finds
 // the largest, and the row(s) matching it are placed at position k.  With no
 // exact tie and no NaN the winner is unique and equals the first-max winner.
-// A tie is a step at which some matrix has more than one winning row: the
-// 16-bit group of the win ballot is then not a power of two (every group has
-// at least one winner while no earlier step tied, so no group field is zero
-// and the SWAR test x & (x - 0x0001000100010001) does not borrow across
-// groups).  Returns false after a tie or a NaN; the caller then reruns the
-// exact serial scan for the whole wave.
+// Both failure cases are detected once, after the last column: a tie places
+// two rows at one position, so the positions of a matrix no longer form a
+// permutation of 0..n-1 (checked with a DPP OR of position bits); a NaN
+// candidate is the largest key of its column (its bit pattern exceeds +inf).
+// Returns false after a tie or a NaN; the caller then reruns the exact serial
+// scan for the whole wave.
 template <int N>
 __device__ __forceinline__ bool pivot_parallel(const float (&w)[Geo<N>::R][N], int (&pos)[Geo<N>::R], int l) {
   using G = Geo<N>;
   bool rem[G::R];
 #pragma unroll
-  for (int h = 0; h < G::R; ++h) { rem[h] = (l + 16 * h) < N; pos[h] = 0; }
-  uint64_t tie = 0;
-  bool bad = false;
+  for (int h = 0; h < G::R; ++h) { rem[h] = (l + 16 * h) < N; pos[h] = 63; }
+  uint32_t top = 0;  // largest key seen: > +inf only if some column's maximum was a NaN
   static_for<0, N>([&](auto K) {
     constexpr int k = decltype(K)::value;
     uint32_t key[G::R];
 #pragma unroll
-    for (int h = 0; h < G::R; ++h) key[h] = rem[h] ? (__builtin_bit_cast(uint32_t, w[h][k]) & 0x7fffffffu) : 0u;
+    for (int h = 0; h < G::R; ++h) key[h] = abs_key(w[h][k], rem[h]);
     uint32_t m = key[0];
 #pragma unroll
     for (int h = 1; h < G::R; ++h) m = max(m, key[h]);
     m = row_max_u32(m);
-    bad = bad || (m > 0x7f800000u);
-    uint64_t any = 0, both = ~uint64_t(0);
+    top = max(top, m);
+    keep_in_vgpr(top);
 #pragma unroll
     for (int h = 0; h < G::R; ++h) {
       const bool win = rem[h] && key[h] == m;
-      const uint64_t b = __builtin_amdgcn_ballot_w64(win);
-      both = (h == 0) ? b : (both & b);
-      any |= b;
       rem[h] = rem[h] && !win;
       pos[h] = win ? k : pos[h];
     }
-    if constexpr (G::R == 1) both = 0;
-    tie |= (any & (any - 0x0001000100010001ull)) | both;
   });
-  return tie == 0 && __builtin_amdgcn_ballot_w64(bad) == 0;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int h = 0; h < G::R; ++h)
+    bits |= ((l + 16 * h) < N && (uint32_t)pos[h] < 32u) ? (1u << pos[h]) : 0u;
+  bits = row_or_u32(bits);
+  constexpr uint32_t full = (N == 32) ? 0xffffffffu : ((1u << N) - 1u);
+  const bool ok = (bits == full) && (top <= 0x7f800000u);
+  return __builtin_amdgcn_ballot_w64(!ok) == 0;
 }
 
 // ------------------------------------------------------ Gauss-Jordan
@@ -320,11 +386,12 @@ __device__ __forceinline__ bool pivot_parallel(const float (&w)[Geo<N>::R][N], i
 // pivots are the diagonal of P*A in order, so the multipliers are those of
 // the reference's no-pivot Doolittle factorisation of P*A.  Step k:
 //   d = S[k][k] (broadcast from lane k%16), r = 1/d
-//   each column c != k:  S[i][c] -= S[i][k] * (S[k][c] * r)   for i != k,
-//     one v_fmac_f32_dpp per element: S[i][k] is broadcast from the lane
-//     owning column k (its entry k is zeroed first so row k is left alone)
-//     and t_c = -S[k][c] * r is the lane's own multiplier
-//   row k is normalised: S[k][c] *= r
+//   g_c = S[k][c] * r, the lane's own multiplier (0 in the lane owning column
+//     k, so column k is left as it is)
+//   every row i:  S[i][c] -= S[i][k] * g_c, one v_fmac_f32_dpp per element
+//     with S[i][k] broadcast (negated) from the lane owning column k; the
+//     result for row k itself is discarded:
+//   row k is set to its normalised values g_c
 //   column k turns into right-hand column k.  It is kept unscaled: the right
 //     part R[i][k] = -S[i][k] / d is sigma_k * S[i][k] with sigma_k = -1/d,
 //     and every later row operation commutes with that column scale; its
@@ -342,37 +409,31 @@ __device__ __forceinline__ void gauss_jordan(float (&w)[Geo<N>::R][N], float (&c
     float r = __builtin_amdgcn_rcpf(d);
     r = __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);  // one Newton step
     const bool pl = (l == kl);                              // this lane owns column k (half kh)
-    float t[G::R];
+    float g[G::R];
 #pragma unroll
-    for (int h = 0; h < G::R; ++h) t[h] = -w[h][k] * r;
-    t[kh] = pl ? 0.0f : t[kh];
+    for (int h = 0; h < G::R; ++h) g[h] = w[h][k] * r;
+    g[kh] = pl ? 0.0f : g[kh];
     cs[kh] = pl ? -r : cs[kh];
-    w[kh][k] = pl ? 0.0f : w[kh][k];
-    if constexpr (G::R == 2) GjBlock<N, kl>::cross(w[1 - kh], w[kh], t[1 - kh]);
-    GjBlock<N, kl>::self(w[kh], t[kh]);
+    if constexpr (G::R == 2) GjBlock<N, kl>::cross(w[1 - kh], w[kh], g[1 - kh]);
+    GjBlock<N, kl>::self(w[kh], g[kh]);
 #pragma unroll
-    for (int h = 0; h < G::R; ++h) w[h][k] *= r;
-    w[kh][k] = pl ? -1.0f : w[kh][k];
+    for (int h = 0; h < G::R; ++h) w[h][k] = g[h];
+    w[kh][k] = pl ? -1.0f : g[kh];
   });
 }
 
 // ------------------------------------------------------------ the kernel
 
+// Phases 2-5 of a tile already staged in LDS: pivot positions, P*A in LDS,
+// then column q of P*A into the registers of lane q.
 template <int N, int PIV>
-__device__ __forceinline__ void invert_tile(float* __restrict__ A, int* __restrict__ piv, int64_t m0, int valid,
-                                            bool vec, float* lds, int lane) {
+__device__ __forceinline__ void tile_front(float (&w)[Geo<N>::R][N], int (&prm)[Geo<N>::R], float* lds, int lane) {
   using G = Geo<N>;
   const int l = lane & 15;
   const int g = lane >> 4;
   float* const mat = lds + g * G::MS;
   int* const perm = reinterpret_cast<int*>(lds + G::PERM) + g * 32;
-  float* const tile = A + m0 * (N * N);
 
-  tile_load<N>(tile, lds, valid, vec, lane);
-  wave_sync();
-
-  float w[G::R][N];
-  int prm[G::R];
   if constexpr (PIV != kPivNone) {
     // 2. original rows l and l+16 (rows past n read row n-1 and stay unused)
 #pragma unroll
@@ -412,6 +473,17 @@ __device__ __forceinline__ void invert_tile(float* __restrict__ A, int* __restri
     prm[h] = (PIV != kPivNone) ? (perm[q] & 31) : q;  // a permutation of 0..n-1; the mask keeps LDS indexing in range
   }
   wave_sync();
+}
+
+// Phases 6-8: (P*A)^-1 in registers, column q written to column perm[q] of the
+// LDS tile, the tile and the pivot vectors stored to HBM.
+template <int N>
+__device__ __forceinline__ void tile_back(float (&w)[Geo<N>::R][N], const int (&prm)[Geo<N>::R], float* __restrict__ A,
+                                          int* __restrict__ piv, int64_t m0, int valid, bool vec, float* lds, int lane) {
+  using G = Geo<N>;
+  const int l = lane & 15;
+  const int g = lane >> 4;
+  float* const mat = lds + g * G::MS;
 
   // 6. (P*A)^-1 = S * diag(cs), column q in lane q
   float cs[G::R];
@@ -429,7 +501,7 @@ __device__ __forceinline__ void invert_tile(float* __restrict__ A, int* __restri
   wave_sync();
 
   // 8. store the inverses and the pivot vectors
-  tile_store<N>(tile, lds, valid, vec, lane);
+  tile_store<N>(A + m0 * (N * N), lds, valid, vec, lane);
   if (piv != nullptr && g < valid) {
 #pragma unroll
     for (int h = 0; h < G::R; ++h)
@@ -438,16 +510,38 @@ __device__ __forceinline__ void invert_tile(float* __restrict__ A, int* __restri
   wave_sync();  // the next tile reuses the LDS region
 }
 
+// Persistent over tiles: each wave strides through the batch.  The next full
+// tile is fetched into registers once the current one has left LDS for
+// registers (after the column read), so its HBM reads overlap this tile's
+// Gauss-Jordan, the longest phase.
+// amdgpu_waves_per_eu(2): at most 256 VGPRs, so two waves share each SIMD
+// (the LDS tile of n = 32 allows no more than that anyway).
 template <int N, int PIV>
-__global__ __launch_bounds__(kWave) void lu_inverse_kernel(float* __restrict__ A, int* __restrict__ piv, int64_t batch,
-                                                           int64_t ntiles, int vec) {
-  __shared__ __attribute__((aligned(16))) float lds[Geo<N>::LDS_FLOATS];
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void lu_inverse_kernel(
+    float* __restrict__ A, int* __restrict__ piv, int64_t batch, int64_t ntiles, int vec) {
+  using G = Geo<N>;
+  __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
   const int lane = threadIdx.x;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  const int64_t stride = gridDim.x;
+  auto full = [&](int64_t t) { return vec != 0 && (t + 1) * kMatricesPerWave <= batch; };
+  TileRegs<N> next;
+  int64_t t = blockIdx.x;
+  bool have = t < ntiles && full(t);
+  if (have) tile_fetch<N>(A + t * kMatricesPerWave * (N * N), next, lane);
+  for (; t < ntiles; t += stride) {
     const int64_t m0 = t * kMatricesPerWave;
     const int64_t left = batch - m0;
     const int valid = left < kMatricesPerWave ? (int)left : kMatricesPerWave;
-    invert_tile<N, PIV>(A, piv, m0, valid, vec != 0, lds, lane);
+    if (have) tile_stage<N>(next, lds, lane);
+    else tile_load_elementwise<N>(A + m0 * (N * N), lds, valid, lane);
+    wave_sync();
+    float w[G::R][N];
+    int prm[G::R];
+    tile_front<N, PIV>(w, prm, lds, lane);
+    const int64_t tn = t + stride;
+    have = tn < ntiles && full(tn);
+    if (have) tile_fetch<N>(A + tn * kMatricesPerWave * (N * N), next, lane);
+    tile_back<N>(w, prm, A, piv, m0, valid, vec != 0, lds, lane);
   }
 }
 

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -38,11 +39,56 @@ const lub::KernelTable& table() {
   return g_table;
 }
 
-// Workgroups are one wave (64 lanes) and own a tile of 4 matrices; the grid
-// strides over tiles.  2^22 workgroups is far above what the 256 CUs hold at
-// once, so the hardware dispatcher balances the work; the cap only keeps the
-// dispatch within the 32-bit grid size for very large batches.
-constexpr int64_t kMaxGrid = int64_t(1) << 22;
+// Workgroups are one wave (64 lanes) and own a tile of 4 matrices at a time;
+// the kernels stride over tiles.  Two launch shapes:
+//   one workgroup per tile (default): the hardware dispatcher starts a new
+//     wave whenever one finishes, so the loads of one wave overlap the
+//     arithmetic of the others on the same SIMD;
+//   persistent: the grid is `waves_per_cu` workgroups per CU (capped by the
+//     kernel's occupancy) and each wave prefetches its next tile into
+//     registers during the current tile's Gauss-Jordan.
+//   several tiles per workgroup: grid = ceil(tiles / T), for small n where a
+//     single tile is too little work to amortise a wave launch.
+// LUB_WAVES_PER_CU=<k> selects the persistent shape with k waves per CU;
+// LUB_TILES_PER_WG=<T> the grouped one (defaults: one workgroup per tile).
+constexpr int64_t kMaxGrid = int64_t(1) << 22;  // keeps grid*64 within 32 bits
+
+int env_int(const char* name) {
+  const char* e = std::getenv(name);
+  return e ? std::atoi(e) : 0;
+}
+
+int env_waves_per_cu() {
+  static int v = env_int("LUB_WAVES_PER_CU");
+  return v;
+}
+
+int env_tiles_per_wg() {
+  static int v = env_int("LUB_TILES_PER_WG");
+  return v;
+}
+
+struct Residency {
+  int device = -1;
+  int waves = 0;  // resident single-wave workgroups for the whole device
+};
+std::mutex g_res_mutex;
+Residency g_res[LUB_MAX_N + 1][lub::kNumPivot];
+
+int resident_workgroups(const void* fn, int n, int pv, int cap_per_cu) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(g_res_mutex);
+  Residency& r = g_res[n][pv];
+  if (r.device != dev) {
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0) != hipSuccess) return 0;
+    r.device = dev;
+    r.waves = std::max(1, cus * std::max(1, std::min(per_cu, cap_per_cu)));
+  }
+  return r.waves;
+}
 
 int validate(const float* A, const int* piv, int n, int64_t batch, int flags) {
   if (n < 1 || n > LUB_MAX_N) return LUB_EINVAL_N;
@@ -64,7 +110,13 @@ int lu_batched_inverse_async64(float* A, int* piv, int n, int64_t batch, int fla
   const void* fn = table().plain[n][flags & LUB_PIVOT_MASK];
   if (fn == nullptr) return LUB_ENODEVICE;
   int64_t ntiles = (batch + 3) / 4;
-  unsigned grid = static_cast<unsigned>(std::min(ntiles, kMaxGrid));
+  int64_t cap = kMaxGrid;
+  if (int t = env_tiles_per_wg(); t > 1) cap = std::min<int64_t>(cap, (ntiles + t - 1) / t);
+  if (int wpc = env_waves_per_cu(); wpc > 0) {
+    cap = resident_workgroups(fn, n, flags & LUB_PIVOT_MASK, wpc);
+    if (cap <= 0) return LUB_ENODEVICE;
+  }
+  unsigned grid = static_cast<unsigned>(std::min<int64_t>(ntiles, cap));
   int vec = (reinterpret_cast<uintptr_t>(A) & 15u) == 0 ? 1 : 0;
   void* args[] = {&A, &piv, &batch, &ntiles, &vec};
   hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, static_cast<hipStream_t>(stream));
