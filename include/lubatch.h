@@ -76,6 +76,18 @@ int lu_batched_inverse_async(float* A, int* piv, int n, int batch, int flags, vo
 /* 64-bit batch form of lu_batched_inverse_async. */
 int lu_batched_inverse_async64(float* A, int* piv, int n, int64_t batch, int flags, void* stream);
 
+/* Device-side inversion check, the reference's verifyInv (parallel_pivot/
+ * verify.hpp:50-103) on the GPU.  For each of `batch` matrices, A (the
+ * original matrices) times X (their computed inverses) is formed in FP32 in
+ * the reference's summation order; the matrix counts as correct when every
+ * entry is within `thr` of the identity (the reference uses 1e-3).
+ *   *d_correct  (device uint64)  += number of correct matrices
+ *   *d_max_bits (device uint32)  = max(*d_max_bits, bits of max |A X - I|)
+ *                                  (non-negative FP32 bit pattern; NaN -> 0x7f800001)
+ * The caller zeroes both before the first call.  Enqueued on `stream`. */
+int lu_batched_residual(const float* A, const float* X, int n, int64_t batch, float thr,
+                        unsigned long long* d_correct, unsigned int* d_max_bits, void* stream);
+
 /* Human-readable text of a status code (static storage). */
 const char* lu_status_string(int status);
 

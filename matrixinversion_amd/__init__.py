@@ -6,11 +6,11 @@ with hand-written HIP kernels; this package binds it with ctypes and carries
 the sweep harness that replaces the reference's run.py.
 """
 from .api import (MAX_N, PIVOT_NAMES, PIVOT_NONE, PIVOT_PARALLEL, PIVOT_SERIAL, PIVOT_TREE, invert,
-                  lu_batched_inverse, version)
+                  lu_batched_inverse, residual, version)
 from .device import DeviceBuffer, Event, Stream, device_count, device_info, synchronize
 
 __all__ = [
     "MAX_N", "PIVOT_NAMES", "PIVOT_NONE", "PIVOT_PARALLEL", "PIVOT_SERIAL", "PIVOT_TREE", "invert",
-    "lu_batched_inverse", "version", "DeviceBuffer", "Event", "Stream", "device_count", "device_info",
+    "lu_batched_inverse", "residual", "version", "DeviceBuffer", "Event", "Stream", "device_count", "device_info",
     "synchronize",
 ]

@@ -21,6 +21,7 @@ ABI = {
     "lu_batched_inverse": (_i, [_vp, _vp, _i, _i, _i]),
     "lu_batched_inverse_async": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "lu_batched_inverse_async64": (_i, [_vp, _vp, _i, _i64, _i, _vp]),
+    "lu_batched_residual": (_i, [_vp, _vp, _i, _i64, ctypes.c_float, _vp, _vp, _vp]),
     "lu_status_string": (ctypes.c_char_p, [_i]),
     "lu_last_error": (ctypes.c_char_p, []),
     "lu_version": (ctypes.c_char_p, []),

@@ -45,6 +45,30 @@ def lu_batched_inverse(A: DeviceBuffer, n: int, batch: int, flags: int = PIVOT_P
               "lu_batched_inverse_async64")
 
 
+def residual(A: DeviceBuffer, X: DeviceBuffer, n: int, batch: int, thr: float = 1e-3,
+             stream: Stream | None = None):
+    """verifyInv on the device: (number of matrices whose A X is within thr of I
+    entry-wise, max |A X - I|)."""
+    out = DeviceBuffer(16)
+    try:
+        out.fill_bytes(0)
+        check(lib().lu_batched_residual(A.ptr, X.ptr, n, batch, thr, out.ptr, out.ptr + 8,
+                                        stream.handle if stream else None), "lu_batched_residual")
+        if stream is not None:
+            stream.synchronize()
+        else:
+            from .device import synchronize
+            synchronize()
+        raw = out.download((4,), np.uint32)
+    finally:
+        out.close()
+    correct = int(raw[0]) | (int(raw[1]) << 32)
+    worst = float(np.array([raw[2]], dtype=np.uint32).view(np.float32)[0])
+    if raw[2] == 0x7f800001:
+        worst = float("nan")
+    return correct, worst
+
+
 def invert(a: np.ndarray, flags: int = PIVOT_PARALLEL, return_pivots: bool = True):
     """Host convenience: upload a (batch, n, n) FP32 array, invert on the GPU,
     download.  Returns (inverses, pivots) or inverses."""

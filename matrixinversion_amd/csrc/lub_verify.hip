@@ -1,0 +1,71 @@
+// lub_verify.hip -- device-side inversion check (the C ABI's lu_batched_residual).
+//
+// The reference checks its results on the host: verifyInv (parallel_pivot/
+// verify.hpp:50-103) forms every entry of A*X in FP32 and counts a matrix as
+// correct when all n*n entries are within 1e-3 of the identity -- an
+// O(batch*n^3) single-threaded loop, the slowest step of each reference run.
+// Here one workgroup checks one matrix: A and X are staged in LDS, thread
+// (i, j) forms (A X)[i][j] in FP32 in the same l order (rounded multiply,
+// rounded add), and the workgroup reduces "all entries within thr" and
+// max |A X - I| (FP32).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lubatch.h"
+
+namespace {
+
+constexpr int kMaxN = LUB_MAX_N;
+
+__global__ __launch_bounds__(1024) void residual_kernel(const float* __restrict__ A, const float* __restrict__ X, int n,
+                                                        int64_t batch, float thr, unsigned long long* __restrict__ correct,
+                                                        unsigned int* __restrict__ max_bits) {
+  __shared__ float a[kMaxN * (kMaxN + 1)];
+  __shared__ float x[kMaxN * (kMaxN + 1)];
+  __shared__ int all_ok;
+  __shared__ unsigned int worst;
+  const int nn = n * n;
+  const int t = threadIdx.x;
+  for (int64_t m = blockIdx.x; m < batch; m += gridDim.x) {
+    if (t == 0) { all_ok = 1; worst = 0u; }
+    if (t < nn) {
+      const int i = t / n, j = t - i * n;
+      a[i * (kMaxN + 1) + j] = A[m * nn + t];
+      x[i * (kMaxN + 1) + j] = X[m * nn + t];
+    }
+    __syncthreads();
+    if (t < nn) {
+      const int i = t / n, j = t - i * n;
+      float r = 0.0f;
+      // separate multiply and add (no FMA contraction), in l order: the
+      // arithmetic of verifyInv's inner loop as written
+      for (int l = 0; l < n; ++l) r = __fadd_rn(r, __fmul_rn(a[i * (kMaxN + 1) + l], x[l * (kMaxN + 1) + j]));
+      const float e = fabsf(r - (i == j ? 1.0f : 0.0f));
+      if (!(e < thr)) atomicAnd(&all_ok, 0);  // NaN fails, as in the reference's '<' test
+      // non-negative floats order like their bit patterns; NaN maps to +inf's pattern + 1
+      const unsigned int b = (e == e) ? __float_as_uint(e) : 0x7f800001u;
+      atomicMax(&worst, b);
+    }
+    __syncthreads();
+    if (t == 0) {
+      if (all_ok) atomicAdd(correct, 1ull);
+      atomicMax(max_bits, worst);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+extern "C" int lu_batched_residual(const float* A, const float* X, int n, int64_t batch, float thr,
+                                   unsigned long long* d_correct, unsigned int* d_max_bits, void* stream) {
+  if (n < 1 || n > LUB_MAX_N) return LUB_EINVAL_N;
+  if (batch < 0) return LUB_EINVAL_BATCH;
+  if (batch == 0) return LUB_OK;
+  if (A == nullptr || X == nullptr || d_correct == nullptr || d_max_bits == nullptr) return LUB_ENULL;
+  const int threads = ((n * n + 63) / 64) * 64;
+  const unsigned grid = static_cast<unsigned>(batch < (int64_t(1) << 20) ? batch : (int64_t(1) << 20));
+  residual_kernel<<<grid, threads, 0, static_cast<hipStream_t>(stream)>>>(A, X, n, batch, thr, d_correct, d_max_bits);
+  return hipGetLastError() == hipSuccess ? LUB_OK : LUB_EHIP;
+}

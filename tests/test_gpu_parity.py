@@ -143,3 +143,24 @@ def test_well_conditioned_bounds(gpu, mode):
         assert np.all(e <= b), f"n={n}: e={e.max():.3e} bound={b[np.argmax(e / b)]:.3e}"
         r = acc.max_abs_axmi(a, x)
         assert r.max() <= 1e-4, f"n={n}: max|AX-I|={r.max():.3e}"
+
+
+def test_device_residual_matches_verify_inv(gpu):
+    """lu_batched_residual counts exactly what the oracle's verifyInv
+    restatement counts (same FP32 summation order), NaN included."""
+    n, batch = 16, 64
+    a = oracle.uniform_matrices(77, batch, n)
+    x, _ = oracle.lu_inverse_f32(a, oracle.PIV_FIRSTMAX)
+    x[3, 2, 2] += np.float32(0.5)
+    x[5, :, :] = np.nan
+    A = gpu.DeviceBuffer.from_array(a)
+    X = gpu.DeviceBuffer.from_array(x)
+    try:
+        print("[case] residual", flush=True)
+        correct, worst = gpu.residual(A, X, n, batch, 1e-3)
+    finally:
+        A.close()
+        X.close()
+    correct_o, worst_o = oracle.verify_inv(a, x)
+    assert correct == correct_o
+    assert np.isnan(worst) or worst >= 0.4

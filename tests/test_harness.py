@@ -1,0 +1,32 @@
+"""The sweep harness (run.py replacement): inputs, keys and sizes (CPU only)."""
+import os
+
+import numpy as np
+
+import oracle
+from matrixinversion_amd import harness
+from tests.conftest import GOLDEN
+
+
+def test_random_inputs_match_the_oracle_generator():
+    """numpy's MT19937 with legacy seeding and the oracle's C MT19937 give the
+    same matrices under the pinned float mapping."""
+    for n, batch, seed in ((3, 5, 1), (32, 7, 42)):
+        assert (harness.make_inputs("random", n, batch, seed) == oracle.uniform_matrices(seed, batch, n)).all()
+        assert (harness.make_inputs("dominant", n, batch, seed) == oracle.dominant_matrices(seed, batch, n)).all()
+
+
+def test_template_inputs_replicate_the_fixture_prefix():
+    """parallel_pivot/luBatchedInplace.cu:44-53, 88-101: the first n*n numbers of
+    the fixture, row-major, copied into every matrix."""
+    path = os.path.join(GOLDEN, "mtrand32_new1.txt")
+    a = harness.make_inputs(f"template:{path}", 5, 3, 0)
+    t = np.loadtxt(path, dtype=np.float32).ravel()[:25].reshape(5, 5)
+    assert a.shape == (3, 5, 5) and all((m == t).all() for m in a)
+
+
+def test_sizes_and_labels():
+    assert harness.parse_sizes("1-4,8,30-32") == [1, 2, 3, 4, 8, 30, 31, 32]
+    assert harness._batch_label(100000) == "100k"
+    assert harness._batch_label(1000000) == "1M"
+    assert harness._batch_label(1234) == "1234"
