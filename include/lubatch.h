@@ -58,6 +58,13 @@ extern "C" {
                                     never reduced -- bit-compatible with that kernel     */
 #define LUB_PIVOT_MASK      0xF
 
+/* Lane layout, bits 4..5 (a tuning override; AUTO picks per n, from the
+ * autotuned table when one is loaded).  Results do not depend on it. */
+#define LUB_LAYOUT_AUTO     0x00
+#define LUB_LAYOUT_ROW16    0x10 /* one matrix per 16-lane DPP row, 4 per wave (n <= 32) */
+#define LUB_LAYOUT_QUAD4    0x20 /* one matrix per 4-lane quad, 16 per wave (n <= 16)   */
+#define LUB_LAYOUT_MASK     0x30
+
 /* ---- entry points ------------------------------------------------------ */
 
 /* Invert `batch` n x n FP32 matrices at device pointer A in place.

@@ -20,6 +20,10 @@ PIVOT_PARALLEL = 0x1
 PIVOT_SERIAL = 0x2
 PIVOT_TREE = 0x3
 
+LAYOUT_AUTO = 0x00
+LAYOUT_ROW16 = 0x10
+LAYOUT_QUAD4 = 0x20
+
 PIVOT_NAMES = {"none": PIVOT_NONE, "parallel": PIVOT_PARALLEL, "serial": PIVOT_SERIAL, "tree": PIVOT_TREE}
 
 MAX_N = 32

@@ -1,30 +1,31 @@
 // lu_kernel.hpp -- gfx950 batched in-place inversion of small FP32 matrices.
 //
-// One wave64 owns a tile of 4 consecutive matrices; each matrix is owned by
-// one 16-lane DPP row of the wave, so every cross-lane broadcast a step needs
-// is a single `row_newbcast` DPP modifier and no LDS barrier is ever shared
-// between waves.  Per tile:
+// A wave64 owns a tile of consecutive matrices; each matrix is owned by a
+// group of L lanes -- a 16-lane DPP row (L = 16, 4 matrices per wave, any n
+// up to 32) or a quad (L = 4, 16 matrices per wave, n up to 16) -- so every
+// cross-lane broadcast a step needs is one DPP modifier (row_newbcast or
+// quad_perm) and no LDS barrier is ever shared between waves.  Per tile:
 //
-//   1. tile load     HBM -> LDS with coalesced dwordx4 loads (the 4 matrices
-//                    are contiguous: 16*n*n bytes)
-//   2. row read      lane l of matrix g holds original rows l and l+16
+//   1. tile load     HBM -> LDS with coalesced dwordx4 loads (the tile's
+//                    matrices are contiguous)
+//   2. row read      lane l of a matrix holds original rows l, l+L, ...
 //   3. pivot search  REF rule (parallel_pivot/luBatchedInplace.cuh:12-44 reads
 //                    column k of the row-permuted *input*; comp_U/comp_L at
 //                    :175-185 have not touched it yet), so the permutation is a
 //                    function of the input alone and is found before any
 //                    arithmetic:
-//                      PARALLEL  4-step DPP max of |a| bit patterns per column,
-//                                ties / NaN detected by ballot count and
-//                                resolved by the SERIAL scan for the wave
+//                      PARALLEL  DPP max of |a| bit patterns per column; ties
+//                                and NaN detected once per tile and resolved
+//                                by the SERIAL scan for the wave
 //                      SERIAL    first-max scan over positions (strict >,
 //                                serial_pivot/luBatchedInplace.cuh:22-36)
 //                      TREE      parallel_pivot's shared-memory tree, exactly
 //   4. permute       rows written back to LDS at their pivot positions: P*A
-//   5. column read   lane q holds column q of P*A
+//   5. column read   lane q holds columns q, q+L, ... of P*A
 //   6. Gauss-Jordan  row operations on P*A, pivot-free (the permutation
 //                    already fixed the pivots, so the multipliers are those of
 //                    the reference's Doolittle LU of P*A), in registers, one
-//                    v_fmac_f32_dpp row_newbcast per update
+//                    v_fmac_f32_dpp per element update
 //   7. unpermute     column p of (P*A)^-1 is column perm[p] of A^-1 (the
 //                    reference's comp_inv folds P into the right-hand side the
 //                    same way, parallel_pivot/luBatchedInplace.cuh:99); it is
@@ -44,29 +45,39 @@
 namespace lub {
 
 constexpr int kWave = 64;
-constexpr int kLanesPerMatrix = 16;
-constexpr int kMatricesPerWave = kWave / kLanesPerMatrix;  // 4
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 enum Pivot : int { kPivNone = 0, kPivParallel = 1, kPivSerial = 2, kPivTree = 3 };
 
-// LDS geometry of one tile (in floats).  Rows whose length is a multiple of 8
-// floats get 4 floats of padding so that 8 consecutive rows start in 8
-// different 16-byte bank groups; matrices with n % 4 == 0 start 16 banks
-// apart so the two matrices of a half-wave column access do not collide.
-// For n % 4 != 0 the tile is stored contiguously (its dwordx4 chunks cross
-// row boundaries).
-template <int N>
+// Tile geometry for n = N with L lanes per matrix (all sizes in floats).
+// Rows whose length is a multiple of 8 floats get 4 floats of padding so that
+// 8 consecutive rows start in 8 different 16-byte bank groups; for n % 4 == 0
+// consecutive matrices start L banks apart so the lane groups of one
+// half-wave column access hit disjoint banks.  For n % 4 != 0 the tile is
+// stored contiguously (its dwordx4 chunks cross row boundaries).
+template <int N, int L>
 struct Geo {
-  static constexpr int R = (N + 15) / 16;  // rows (or columns) per lane
+  static_assert(L == 16 || L == 4, "a matrix is owned by a DPP row or a quad");
+  static_assert(N >= 1 && N <= (L == 16 ? 32 : 16), "n too large for this lane grouping");
+  static constexpr int MPW = kWave / L;          // matrices per wave (tile)
+  static constexpr int R = (N + L - 1) / L;      // rows (columns) per lane
   static constexpr int LD = (N % 8 == 0) ? N + 4 : N;
-  static constexpr int MS = (N % 4 == 0) ? ((N * LD + 31) / 32) * 32 + 16 : N * N;
-  static constexpr int TILE = kMatricesPerWave * MS;
-  static constexpr int PERM = TILE;                     // int32[4][32] after the tile
-  static constexpr int LDS_FLOATS = TILE + kMatricesPerWave * 32;
+  static constexpr int MS = (N % 4 == 0) ? ((N * LD + 31) / 32) * 32 + L : N * N;
+  static constexpr int TILE = MPW * MS;
+  static constexpr int PERM = TILE;               // int32[MPW][32] after the tile
+  static constexpr int LDS_FLOATS = TILE + MPW * 32;
 };
+
+// Per-lane register blocks of a matrix: R columns (or rows) of n floats, and
+// one int / float per column.
+template <int N, int L>
+using ColBlock = float[Geo<N, L>::R][N];
+template <int N, int L>
+using LaneInts = int[Geo<N, L>::R];
+template <int N, int L>
+using LaneFloats = float[Geo<N, L>::R];
 
 // ------------------------------------------------------------------ helpers
 
@@ -88,12 +99,34 @@ __device__ __forceinline__ float dpp_f32(float x) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, true));
 }
 
-// max over the 16 lanes of a DPP row (quad xor 1, quad xor 2, half mirror, mirror).
-__device__ __forceinline__ uint32_t row_max_u32(uint32_t m) {
-  m = max(m, dpp_u32<0xB1>(m));   // quad_perm [1,0,3,2]
-  m = max(m, dpp_u32<0x4E>(m));   // quad_perm [2,3,0,1]
-  m = max(m, dpp_u32<0x141>(m));  // row_half_mirror
-  m = max(m, dpp_u32<0x140>(m));  // row_mirror
+// DPP control that broadcasts lane KL of each lane group to the whole group.
+template <int L, int KL>
+constexpr int bcast_ctrl() {
+  return L == 16 ? 0x150 + KL   // row_newbcast:KL
+                 : KL * 0x55;   // quad_perm:[KL,KL,KL,KL]
+}
+
+// max / or over the L lanes of a group (quad xor 1, quad xor 2, then for
+// 16-lane rows half mirror and mirror).
+template <int L>
+__device__ __forceinline__ uint32_t group_max_u32(uint32_t m) {
+  m = max(m, dpp_u32<0xB1>(m));  // quad_perm [1,0,3,2]
+  m = max(m, dpp_u32<0x4E>(m));  // quad_perm [2,3,0,1]
+  if constexpr (L == 16) {
+    m = max(m, dpp_u32<0x141>(m));  // row_half_mirror
+    m = max(m, dpp_u32<0x140>(m));  // row_mirror
+  }
+  return m;
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t group_or_u32(uint32_t m) {
+  m |= dpp_u32<0xB1>(m);
+  m |= dpp_u32<0x4E>(m);
+  if constexpr (L == 16) {
+    m |= dpp_u32<0x141>(m);
+    m |= dpp_u32<0x140>(m);
+  }
   return m;
 }
 
@@ -110,15 +143,6 @@ __device__ __forceinline__ uint32_t abs_key(float x, bool live) {
 // reduction from being re-associated into a tree over all its inputs (which
 // would keep every step's value live until the end).
 __device__ __forceinline__ void keep_in_vgpr(uint32_t& v) { asm volatile("" : "+v"(v)); }
-
-// bitwise OR over the 16 lanes of a DPP row.
-__device__ __forceinline__ uint32_t row_or_u32(uint32_t m) {
-  m |= dpp_u32<0xB1>(m);
-  m |= dpp_u32<0x4E>(m);
-  m |= dpp_u32<0x141>(m);
-  m |= dpp_u32<0x140>(m);
-  return m;
-}
 
 // Orders this wave's LDS accesses (the tile region is private to the wave;
 // the workgroup is one wave, so the barrier is free).
@@ -162,28 +186,31 @@ __device__ __forceinline__ void lds_row_write(float* p, const float (&w)[N]) {
 // ------------------------------------------------------------- tile I/O
 
 // HBM -> LDS.  Full tiles of a 16-byte aligned batch move as dwordx4 chunks
-// (chunk c = floats 4c..4c+3 of the 4*n*n-float tile), fetched into registers
-// one tile ahead (TileRegs) and staged into LDS when the tile's turn comes.
-// The last, partial tile and unaligned batches move element-wise, and matrices
-// past the end of the batch are filled with the identity so every lane
-// computes on finite data.
-template <int N>
+// (chunk c = floats 4c..4c+3 of the tile), fetched into registers (TileRegs)
+// and staged into LDS.  The last, partial tile and unaligned batches move
+// element-wise, and matrices past the end of the batch are filled with the
+// identity so every lane computes on finite data.
+template <int N, int L>
 struct TileRegs {
-  static constexpr int CH = N * N;  // dwordx4 chunks per tile (4*n*n floats / 4)
+  static constexpr int CH = Geo<N, L>::MPW * N * N / 4;  // dwordx4 chunks per tile
   static constexpr int IT = (CH + kWave - 1) / kWave;
   f32x4 v[IT];
 };
 
-// LDS offset (floats) of dwordx4 chunk c = lane + 64*it of the tile.  When n
-// divides 256 (one wave iteration covers whole rows) it splits into a per-lane
+// LDS offset (floats) of dwordx4 chunk c = lane + 64*it of the tile.  When
+// one wave iteration (256 floats) covers whole rows and whole matrices (or a
+// whole number of iterations covers one matrix), it splits into a per-lane
 // base and a compile-time offset per iteration, so the staging loop needs one
 // address register instead of one per chunk.
 template <int N>
+constexpr bool kChunkSplits = (N % 4 == 0) && (256 % N == 0) && ((N * N) % 256 == 0 || 256 % (N * N) == 0);
+
+template <int N, int L>
 __device__ __forceinline__ int chunk_lds_base(int lane) {
-  using G = Geo<N>;
+  using G = Geo<N, L>;
   constexpr int NN = N * N;
-  if constexpr (256 % N == 0 && N >= 8) {
-    const int f = 4 * lane;                        // within the first iteration
+  if constexpr (kChunkSplits<N>) {
+    const int f = 4 * lane;  // within the first iteration
     const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
     return m * G::MS + i * G::LD + j;
   } else {
@@ -191,15 +218,14 @@ __device__ __forceinline__ int chunk_lds_base(int lane) {
   }
 }
 
-template <int N>
+template <int N, int L>
 __device__ __forceinline__ int chunk_lds_offset(int base, int lane, int it) {
-  using G = Geo<N>;
+  using G = Geo<N, L>;
   constexpr int NN = N * N;
-  if constexpr (256 % N == 0 && N >= 8) {
-    // iteration it adds 256 floats: 256/N rows, and whole matrices past NN
-    const int f0 = 256 * it;
+  if constexpr (kChunkSplits<N>) {
+    const int f0 = 256 * it;  // iteration it adds 256 floats: whole rows / matrices
     const int m = f0 / NN, e = f0 - m * NN, i = e / N;
-    return base + m * G::MS + i * G::LD;            // it is a compile-time constant after unrolling
+    return base + m * G::MS + i * G::LD;  // compile-time after unrolling
   } else {
     const int f = 4 * (lane + kWave * it);
     const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
@@ -207,9 +233,9 @@ __device__ __forceinline__ int chunk_lds_offset(int base, int lane, int it) {
   }
 }
 
-template <int N>
-__device__ __forceinline__ void tile_fetch(const float* __restrict__ src, TileRegs<N>& r, int lane) {
-  using T = TileRegs<N>;
+template <int N, int L>
+__device__ __forceinline__ void tile_fetch(const float* __restrict__ src, TileRegs<N, L>& r, int lane) {
+  using T = TileRegs<N, L>;
 #pragma unroll
   for (int it = 0; it < T::IT; ++it) {
     const int c = lane + kWave * it;
@@ -217,41 +243,41 @@ __device__ __forceinline__ void tile_fetch(const float* __restrict__ src, TileRe
   }
 }
 
-template <int N>
-__device__ __forceinline__ void tile_stage(const TileRegs<N>& r, float* lds, int lane) {
-  using T = TileRegs<N>;
-  const int base = chunk_lds_base<N>(lane);
+template <int N, int L>
+__device__ __forceinline__ void tile_stage(const TileRegs<N, L>& r, float* lds, int lane) {
+  using T = TileRegs<N, L>;
+  const int base = chunk_lds_base<N, L>(lane);
 #pragma unroll
   for (int it = 0; it < T::IT; ++it) {
     const int c = lane + kWave * it;
     if (T::CH % kWave == 0 || c < T::CH)
-      *reinterpret_cast<f32x4*>(lds + chunk_lds_offset<N>(base, lane, it)) = r.v[it];
+      *reinterpret_cast<f32x4*>(lds + chunk_lds_offset<N, L>(base, lane, it)) = r.v[it];
   }
 }
 
-template <int N>
+template <int N, int L>
 __device__ __forceinline__ void tile_load_elementwise(const float* __restrict__ src, float* lds, int valid, int lane) {
-  using G = Geo<N>;
+  using G = Geo<N, L>;
   constexpr int NN = N * N;
-  for (int f = lane; f < kMatricesPerWave * NN; f += kWave) {
+  for (int f = lane; f < G::MPW * NN; f += kWave) {
     const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
     lds[m * G::MS + i * G::LD + j] = (m < valid) ? src[f] : (i == j ? 1.0f : 0.0f);
   }
 }
 
 // LDS -> HBM, the mirror of the load; only the `valid` leading matrices are stored.
-template <int N>
+template <int N, int L>
 __device__ __forceinline__ void tile_store(float* __restrict__ dst, const float* lds, int valid, bool vec, int lane) {
-  using G = Geo<N>;
-  using T = TileRegs<N>;
+  using G = Geo<N, L>;
+  using T = TileRegs<N, L>;
   constexpr int NN = N * N;
-  if (vec && valid == kMatricesPerWave) {
-    const int base = chunk_lds_base<N>(lane);
+  if (vec && valid == G::MPW) {
+    const int base = chunk_lds_base<N, L>(lane);
 #pragma unroll
     for (int it = 0; it < T::IT; ++it) {
       const int c = lane + kWave * it;
       if (T::CH % kWave == 0 || c < T::CH) {
-        f32x4 v = *reinterpret_cast<const f32x4*>(lds + chunk_lds_offset<N>(base, lane, it));
+        f32x4 v = *reinterpret_cast<const f32x4*>(lds + chunk_lds_offset<N, L>(base, lane, it));
         __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(dst) + c);
       }
     }
@@ -270,16 +296,17 @@ __device__ __forceinline__ void tile_store(float* __restrict__ dst, const float*
 // lane walks positions k..n-1 with strict '>' (serial_pivot/luBatchedInplace.cuh:
 // 25-34) and swaps (:134-139).  A NaN at position k is never beaten; a NaN
 // elsewhere never wins -- the reference's comparison semantics.
-template <int N>
-__device__ __forceinline__ void pivot_serial(const float (&w)[Geo<N>::R][N], int (&pos)[Geo<N>::R], float* col, int l) {
-  using G = Geo<N>;
+template <int N, int L>
+__device__ __forceinline__ void pivot_serial(const ColBlock<N, L>& w, LaneInts<N, L>& pos, float* col,
+                                             int l) {
+  using G = Geo<N, L>;
 #pragma unroll
-  for (int h = 0; h < G::R; ++h) pos[h] = l + 16 * h;
+  for (int h = 0; h < G::R; ++h) pos[h] = l + L * h;
   static_for<0, N>([&](auto K) {
     constexpr int k = decltype(K)::value;
 #pragma unroll
     for (int h = 0; h < G::R; ++h)
-      if (l + 16 * h < N && pos[h] >= k) col[pos[h]] = fabsf(w[h][k]);
+      if (l + L * h < N && pos[h] >= k) col[pos[h]] = fabsf(w[h][k]);
     wave_sync();
     int best = k;
     float bv = col[k];
@@ -299,16 +326,17 @@ __device__ __forceinline__ void pivot_serial(const float (&w)[Geo<N>::R][N], int
 // (|a_kk|, k) and takes position k+1+t when strictly larger; the tree then
 // walks stride = n/2, n/4, ..., 1 keeping the left slot unless it is strictly
 // smaller.  For n not a power of two some slots never reach slot 0, as there.
-template <int N>
-__device__ __forceinline__ void pivot_tree(const float (&w)[Geo<N>::R][N], int (&pos)[Geo<N>::R], float* col, int l) {
-  using G = Geo<N>;
+template <int N, int L>
+__device__ __forceinline__ void pivot_tree(const ColBlock<N, L>& w, LaneInts<N, L>& pos, float* col,
+                                           int l) {
+  using G = Geo<N, L>;
 #pragma unroll
-  for (int h = 0; h < G::R; ++h) pos[h] = l + 16 * h;
+  for (int h = 0; h < G::R; ++h) pos[h] = l + L * h;
   static_for<0, N>([&](auto K) {
     constexpr int k = decltype(K)::value;
 #pragma unroll
     for (int h = 0; h < G::R; ++h)
-      if (l + 16 * h < N && pos[h] >= k) col[pos[h]] = fabsf(w[h][k]);
+      if (l + L * h < N && pos[h] >= k) col[pos[h]] = fabsf(w[h][k]);
     wave_sync();
     const float vk = col[k];
     float sv[N];
@@ -335,21 +363,21 @@ __device__ __forceinline__ void pivot_tree(const float (&w)[Geo<N>::R][N], int (
 }
 
 // Cross-lane argmax: per column, each still-unplaced row offers the bit
-// pattern of |a| (monotone in |a| for non-NaN values), a 4-step DPP max finds
-// the largest, and the row(s) matching it are placed at position k.  With no
-// exact tie and no NaN the winner is unique and equals the first-max winner.
-// Both failure cases are detected once, after the last column: a tie places
-// two rows at one position, so the positions of a matrix no longer form a
-// permutation of 0..n-1 (checked with a DPP OR of position bits); a NaN
-// candidate is the largest key of its column (its bit pattern exceeds +inf).
-// Returns false after a tie or a NaN; the caller then reruns the exact serial
-// scan for the whole wave.
-template <int N>
-__device__ __forceinline__ bool pivot_parallel(const float (&w)[Geo<N>::R][N], int (&pos)[Geo<N>::R], int l) {
-  using G = Geo<N>;
+// pattern of |a| (monotone in |a| for non-NaN values), a DPP max over the
+// lane group finds the largest, and the row(s) matching it are placed at
+// position k.  With no exact tie and no NaN the winner is unique and equals
+// the first-max winner.  Both failure cases are detected once, after the last
+// column: a tie places two rows at one position, so the positions of a
+// matrix no longer form a permutation of 0..n-1 (checked with a DPP OR of
+// position bits); a NaN candidate is the largest key of its column (its bit
+// pattern exceeds +inf).  Returns false after a tie or a NaN; the caller then
+// reruns the exact serial scan for the whole wave.
+template <int N, int L>
+__device__ __forceinline__ bool pivot_parallel(const ColBlock<N, L>& w, LaneInts<N, L>& pos, int l) {
+  using G = Geo<N, L>;
   bool rem[G::R];
 #pragma unroll
-  for (int h = 0; h < G::R; ++h) { rem[h] = (l + 16 * h) < N; pos[h] = 63; }
+  for (int h = 0; h < G::R; ++h) { rem[h] = (l + L * h) < N; pos[h] = 63; }
   uint32_t top = 0;  // largest key seen: > +inf only if some column's maximum was a NaN
   static_for<0, N>([&](auto K) {
     constexpr int k = decltype(K)::value;
@@ -359,7 +387,7 @@ __device__ __forceinline__ bool pivot_parallel(const float (&w)[Geo<N>::R][N], i
     uint32_t m = key[0];
 #pragma unroll
     for (int h = 1; h < G::R; ++h) m = max(m, key[h]);
-    m = row_max_u32(m);
+    m = group_max_u32<L>(m);
     top = max(top, m);
     keep_in_vgpr(top);
 #pragma unroll
@@ -372,8 +400,8 @@ __device__ __forceinline__ bool pivot_parallel(const float (&w)[Geo<N>::R][N], i
   uint32_t bits = 0;
 #pragma unroll
   for (int h = 0; h < G::R; ++h)
-    bits |= ((l + 16 * h) < N && (uint32_t)pos[h] < 32u) ? (1u << pos[h]) : 0u;
-  bits = row_or_u32(bits);
+    bits |= ((l + L * h) < N && (uint32_t)pos[h] < 32u) ? (1u << pos[h]) : 0u;
+  bits = group_or_u32<L>(bits);
   constexpr uint32_t full = (N == 32) ? 0xffffffffu : ((1u << N) - 1u);
   const bool ok = (bits == full) && (top <= 0x7f800000u);
   return __builtin_amdgcn_ballot_w64(!ok) == 0;
@@ -382,10 +410,10 @@ __device__ __forceinline__ bool pivot_parallel(const float (&w)[Geo<N>::R][N], i
 // ------------------------------------------------------ Gauss-Jordan
 
 // In-place Gauss-Jordan with row operations on S = P*A held by columns: lane l
-// of the DPP row holds columns l and l+16 (w[h][i] = S[i][l + 16h]).  The
+// of the group holds columns l, l+L, ... (w[h][i] = S[i][l + L*h]).  The
 // pivots are the diagonal of P*A in order, so the multipliers are those of
 // the reference's no-pivot Doolittle factorisation of P*A.  Step k:
-//   d = S[k][k] (broadcast from lane k%16), r = 1/d
+//   d = S[k][k] (broadcast from lane k%L of the group), r = 1/d
 //   g_c = S[k][c] * r, the lane's own multiplier (0 in the lane owning column
 //     k, so column k is left as it is)
 //   every row i:  S[i][c] -= S[i][k] * g_c, one v_fmac_f32_dpp per element
@@ -397,25 +425,30 @@ __device__ __forceinline__ bool pivot_parallel(const float (&w)[Geo<N>::R][N], i
 //     and every later row operation commutes with that column scale; its
 //     pivot entry becomes R[k][k] / sigma_k = -1.
 // At the end (P*A)^-1 = S * diag(sigma); sigma of the lane's columns is cs[].
-template <int N>
-__device__ __forceinline__ void gauss_jordan(float (&w)[Geo<N>::R][N], float (&cs)[Geo<N>::R], int l) {
-  using G = Geo<N>;
+template <int N, int L>
+__device__ __forceinline__ void gauss_jordan(ColBlock<N, L>& w, LaneFloats<N, L>& cs, int l) {
+  using G = Geo<N, L>;
 #pragma unroll
   for (int h = 0; h < G::R; ++h) cs[h] = 1.0f;
   static_for<0, N>([&](auto K) {
     constexpr int k = decltype(K)::value;
-    constexpr int kl = k & 15, kh = k >> 4;
-    const float d = dpp_f32<0x150 + kl>(w[kh][k]);
+    constexpr int kl = k % L, kh = k / L;
+    const float d = dpp_f32<bcast_ctrl<L, kl>()>(w[kh][k]);
     float r = __builtin_amdgcn_rcpf(d);
     r = __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);  // one Newton step
-    const bool pl = (l == kl);                              // this lane owns column k (half kh)
+    const bool pl = (l == kl);                              // this lane owns column k (set kh)
     float g[G::R];
 #pragma unroll
     for (int h = 0; h < G::R; ++h) g[h] = w[h][k] * r;
     g[kh] = pl ? 0.0f : g[kh];
     cs[kh] = pl ? -r : cs[kh];
-    if constexpr (G::R == 2) GjBlock<N, kl>::cross(w[1 - kh], w[kh], g[1 - kh]);
-    GjBlock<N, kl>::self(w[kh], g[kh]);
+    static_for<0, G::R>([&](auto H) {
+      constexpr int h = decltype(H)::value;
+      // (h in the template argument keeps the call dependent, so it is only
+      // instantiated for h != kh; cross blocks exist only where R > 1)
+      if constexpr (h != kh) GjBlock<N + 0 * h, L, kl>::cross(w[h], w[kh], g[h]);
+    });
+    GjBlock<N, L, kl>::self(w[kh], g[kh]);
 #pragma unroll
     for (int h = 0; h < G::R; ++h) w[h][k] = g[h];
     w[kh][k] = pl ? -1.0f : g[kh];
@@ -425,40 +458,41 @@ __device__ __forceinline__ void gauss_jordan(float (&w)[Geo<N>::R][N], float (&c
 // ------------------------------------------------------------ the kernel
 
 // Phases 2-5 of a tile already staged in LDS: pivot positions, P*A in LDS,
-// then column q of P*A into the registers of lane q.
-template <int N, int PIV>
-__device__ __forceinline__ void tile_front(float (&w)[Geo<N>::R][N], int (&prm)[Geo<N>::R], float* lds, int lane) {
-  using G = Geo<N>;
-  const int l = lane & 15;
-  const int g = lane >> 4;
+// then columns q, q+L, ... of P*A into the registers of lane q.
+template <int N, int L, int PIV>
+__device__ __forceinline__ void tile_front(ColBlock<N, L>& w, LaneInts<N, L>& prm, float* lds,
+                                           int lane) {
+  using G = Geo<N, L>;
+  const int l = lane % L;
+  const int g = lane / L;
   float* const mat = lds + g * G::MS;
   int* const perm = reinterpret_cast<int*>(lds + G::PERM) + g * 32;
 
   if constexpr (PIV != kPivNone) {
-    // 2. original rows l and l+16 (rows past n read row n-1 and stay unused)
+    // 2. original rows l, l+L, ... (rows past n read row n-1 and stay unused)
 #pragma unroll
-    for (int h = 0; h < G::R; ++h) lds_row_read<N>(mat + min(l + 16 * h, N - 1) * G::LD, w[h]);
+    for (int h = 0; h < G::R; ++h) lds_row_read<N>(mat + min(l + L * h, N - 1) * G::LD, w[h]);
     // 3. pivot positions of this lane's rows
     int pos[G::R];
     if constexpr (PIV == kPivParallel) {
-      if (!pivot_parallel<N>(w, pos, l)) {
+      if (!pivot_parallel<N, L>(w, pos, l)) {
         wave_sync();
-        pivot_serial<N>(w, pos, mat, l);
+        pivot_serial<N, L>(w, pos, mat, l);
       }
     } else if constexpr (PIV == kPivSerial) {
       wave_sync();
-      pivot_serial<N>(w, pos, mat, l);
+      pivot_serial<N, L>(w, pos, mat, l);
     } else {
       wave_sync();
-      pivot_tree<N>(w, pos, mat, l);
+      pivot_tree<N, L>(w, pos, mat, l);
     }
     wave_sync();
     // 4. P*A: row rho goes to position pos; perm[pos] = rho
 #pragma unroll
     for (int h = 0; h < G::R; ++h) {
-      if (l + 16 * h < N) {
+      if (l + L * h < N) {
         lds_row_write<N>(mat + pos[h] * G::LD, w[h]);
-        perm[pos[h]] = l + 16 * h;
+        perm[pos[h]] = l + L * h;
       }
     }
     wave_sync();
@@ -467,7 +501,7 @@ __device__ __forceinline__ void tile_front(float (&w)[Geo<N>::R][N], int (&prm)[
   // 5. column q of P*A -> lane q
 #pragma unroll
   for (int h = 0; h < G::R; ++h) {
-    const int q = min(l + 16 * h, N - 1);
+    const int q = min(l + L * h, N - 1);
 #pragma unroll
     for (int p = 0; p < N; ++p) w[h][p] = mat[p * G::LD + q];
     prm[h] = (PIV != kPivNone) ? (perm[q] & 31) : q;  // a permutation of 0..n-1; the mask keeps LDS indexing in range
@@ -477,22 +511,23 @@ __device__ __forceinline__ void tile_front(float (&w)[Geo<N>::R][N], int (&prm)[
 
 // Phases 6-8: (P*A)^-1 in registers, column q written to column perm[q] of the
 // LDS tile, the tile and the pivot vectors stored to HBM.
-template <int N>
-__device__ __forceinline__ void tile_back(float (&w)[Geo<N>::R][N], const int (&prm)[Geo<N>::R], float* __restrict__ A,
-                                          int* __restrict__ piv, int64_t m0, int valid, bool vec, float* lds, int lane) {
-  using G = Geo<N>;
-  const int l = lane & 15;
-  const int g = lane >> 4;
+template <int N, int L>
+__device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L>& prm,
+                                          float* __restrict__ A, int* __restrict__ piv, int64_t m0, int valid,
+                                          bool vec, float* lds, int lane) {
+  using G = Geo<N, L>;
+  const int l = lane % L;
+  const int g = lane / L;
   float* const mat = lds + g * G::MS;
 
   // 6. (P*A)^-1 = S * diag(cs), column q in lane q
   float cs[G::R];
-  gauss_jordan<N>(w, cs, l);
+  gauss_jordan<N, L>(w, cs, l);
 
   // 7. column q of (P*A)^-1 is column perm[q] of A^-1
 #pragma unroll
   for (int h = 0; h < G::R; ++h) {
-    if (l + 16 * h < N) {
+    if (l + L * h < N) {
       float* const colp = mat + prm[h];
 #pragma unroll
       for (int i = 0; i < N; ++i) colp[i * G::LD] = w[h][i] * cs[h];
@@ -501,47 +536,47 @@ __device__ __forceinline__ void tile_back(float (&w)[Geo<N>::R][N], const int (&
   wave_sync();
 
   // 8. store the inverses and the pivot vectors
-  tile_store<N>(A + m0 * (N * N), lds, valid, vec, lane);
+  tile_store<N, L>(A + m0 * (N * N), lds, valid, vec, lane);
   if (piv != nullptr && g < valid) {
 #pragma unroll
     for (int h = 0; h < G::R; ++h)
-      if (l + 16 * h < N) piv[(m0 + g) * N + l + 16 * h] = prm[h];
+      if (l + L * h < N) piv[(m0 + g) * N + l + L * h] = prm[h];
   }
   wave_sync();  // the next tile reuses the LDS region
 }
 
-// Persistent over tiles: each wave strides through the batch.  The next full
-// tile is fetched into registers once the current one has left LDS for
-// registers (after the column read), so its HBM reads overlap this tile's
-// Gauss-Jordan, the longest phase.
+// Each workgroup (one wave) strides over tiles.  The next full tile is
+// fetched into registers once the current one has left LDS for registers
+// (after the column read), so with several tiles per workgroup its HBM reads
+// overlap this tile's Gauss-Jordan, the longest phase.
 // amdgpu_waves_per_eu(2): at most 256 VGPRs, so two waves share each SIMD
 // (the LDS tile of n = 32 allows no more than that anyway).
-template <int N, int PIV>
+template <int N, int L, int PIV>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void lu_inverse_kernel(
     float* __restrict__ A, int* __restrict__ piv, int64_t batch, int64_t ntiles, int vec) {
-  using G = Geo<N>;
+  using G = Geo<N, L>;
   __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
   const int lane = threadIdx.x;
   const int64_t stride = gridDim.x;
-  auto full = [&](int64_t t) { return vec != 0 && (t + 1) * kMatricesPerWave <= batch; };
-  TileRegs<N> next;
+  auto full = [&](int64_t t) { return vec != 0 && (t + 1) * G::MPW <= batch; };
+  TileRegs<N, L> next;
   int64_t t = blockIdx.x;
   bool have = t < ntiles && full(t);
-  if (have) tile_fetch<N>(A + t * kMatricesPerWave * (N * N), next, lane);
+  if (have) tile_fetch<N, L>(A + t * G::MPW * (N * N), next, lane);
   for (; t < ntiles; t += stride) {
-    const int64_t m0 = t * kMatricesPerWave;
+    const int64_t m0 = t * G::MPW;
     const int64_t left = batch - m0;
-    const int valid = left < kMatricesPerWave ? (int)left : kMatricesPerWave;
-    if (have) tile_stage<N>(next, lds, lane);
-    else tile_load_elementwise<N>(A + m0 * (N * N), lds, valid, lane);
+    const int valid = left < G::MPW ? (int)left : G::MPW;
+    if (have) tile_stage<N, L>(next, lds, lane);
+    else tile_load_elementwise<N, L>(A + m0 * (N * N), lds, valid, lane);
     wave_sync();
     float w[G::R][N];
     int prm[G::R];
-    tile_front<N, PIV>(w, prm, lds, lane);
+    tile_front<N, L, PIV>(w, prm, lds, lane);
     const int64_t tn = t + stride;
     have = tn < ntiles && full(tn);
-    if (have) tile_fetch<N>(A + tn * kMatricesPerWave * (N * N), next, lane);
-    tile_back<N>(w, prm, A, piv, m0, valid, vec != 0, lds, lane);
+    if (have) tile_fetch<N, L>(A + tn * G::MPW * (N * N), next, lane);
+    tile_back<N, L>(w, prm, A, piv, m0, valid, vec != 0, lds, lane);
   }
 }
 

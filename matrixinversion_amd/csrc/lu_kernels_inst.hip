@@ -1,5 +1,6 @@
-// lu_kernels_inst.hip -- instantiates lu_inverse_kernel<N, PIV> for
-// N in [LUB_N_LO, LUB_N_HI] and registers the host handles.  The build
+// lu_kernels_inst.hip -- instantiates lu_inverse_kernel<N, L, PIV> for
+// N in [LUB_N_LO, LUB_N_HI] (L = 16 always, L = 4 for N <= 16) and registers
+// the host handles.  The build
 // compiles this file once per size range so the (large, fully unrolled)
 // kernels compile in parallel.
 #include "lu_kernel.hpp"
@@ -15,12 +16,19 @@
 
 namespace {
 
+template <int N, int L>
+void register_layout(lub::KernelTable* t, int layout) {
+  t->plain[N][lub::kPivNone][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivNone>);
+  t->plain[N][lub::kPivParallel][layout] =
+      reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivParallel>);
+  t->plain[N][lub::kPivSerial][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivSerial>);
+  t->plain[N][lub::kPivTree][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivTree>);
+}
+
 template <int N>
 void register_size(lub::KernelTable* t) {
-  t->plain[N][lub::kPivNone] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, lub::kPivNone>);
-  t->plain[N][lub::kPivParallel] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, lub::kPivParallel>);
-  t->plain[N][lub::kPivSerial] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, lub::kPivSerial>);
-  t->plain[N][lub::kPivTree] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, lub::kPivTree>);
+  register_layout<N, 16>(t, 0);
+  if constexpr (N <= 16) register_layout<N, 4>(t, 1);
 }
 
 template <int N>

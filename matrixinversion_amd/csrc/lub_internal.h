@@ -7,13 +7,16 @@
 namespace lub {
 
 constexpr int kMaxN = 32;
-constexpr int kNumPivot = 4;  // NONE, PARALLEL, SERIAL, TREE
+constexpr int kNumPivot = 4;   // NONE, PARALLEL, SERIAL, TREE
+constexpr int kNumLayout = 2;  // 0: 16 lanes per matrix (n <= 32), 1: 4 lanes per matrix (n <= 16)
+constexpr int kLayoutLanes[kNumLayout] = {16, 4};
+constexpr int kLayoutMaxN[kNumLayout] = {32, 16};
 
 // Host handles of the launchable kernels, filled by the per-size-range
-// translation units (each one instantiates lu_inverse_kernel<N, PIV> for its
-// range of N and registers the host stubs here).
+// translation units (each one instantiates lu_inverse_kernel<N, L, PIV> for
+// its range of N and registers the host stubs here).
 struct KernelTable {
-  const void* plain[kMaxN + 1][kNumPivot];
+  const void* plain[kMaxN + 1][kNumPivot][kNumLayout];
 };
 
 }  // namespace lub

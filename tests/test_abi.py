@@ -48,7 +48,9 @@ def test_host_side_validation_codes():
     assert lib.lu_batched_inverse(fake, None, 33, 1, 1) == 1
     assert lib.lu_batched_inverse(fake, None, 4, -1, 1) == 2         # LUB_EINVAL_BATCH
     assert lib.lu_batched_inverse(None, None, 4, 5, 1) == 3          # LUB_ENULL
-    assert lib.lu_batched_inverse(fake, None, 4, 1, 0x10) == 4       # LUB_EFLAGS
+    assert lib.lu_batched_inverse(fake, None, 4, 1, 0x40) == 4       # LUB_EFLAGS
+    assert lib.lu_batched_inverse(fake, None, 4, 1, 0x31) == 4       # both layouts
+    assert lib.lu_batched_inverse(fake, None, 17, 1, 0x21) == 4      # quads hold n <= 16
     assert lib.lu_batched_inverse(fake, None, 4, 1, 0x7) == 4
     assert lib.lu_batched_inverse(fake + 2, None, 4, 1, 1) == 7      # LUB_EALIGN
     assert lib.lu_batched_inverse(None, None, 4, 0, 1) == 0          # empty batch: nothing to do

@@ -27,8 +27,12 @@ def _inputs(mode, n, batch, seed):
     return oracle.uniform_matrices(seed, batch, n)
 
 
-def _check(mia, a, mode, tag):
+LAYOUTS = {"row16": (0x10, 32), "quad4": (0x20, 16)}
+
+
+def _check(mia, a, mode, tag, layout_flag=0):
     flags, rule = MODES[mode]
+    flags |= layout_flag
     print(f"[case] {tag} mode={mode} shape={a.shape}", flush=True)
     x, piv = mia.invert(a, flags)
     xo, po = oracle.lu_inverse_f32(a, rule)
@@ -43,13 +47,16 @@ def _check(mia, a, mode, tag):
     return x, piv, e_gpu, e_orc
 
 
+@pytest.mark.parametrize("layout", ["row16", "quad4"])
 @pytest.mark.parametrize("mode", ["parallel", "none", "serial", "tree"])
-def test_all_sizes(gpu, mode):
-    """n = 1..32 with a 37-matrix batch (9 full tiles and a 1-matrix tail tile)."""
+def test_all_sizes(gpu, mode, layout):
+    """Every n of the layout; 37 matrices is a partial last tile in both layouts
+    (9 tiles + 1 of 4, or 2 tiles + 5 of 16)."""
+    lf, nmax = LAYOUTS[layout]
     rows = []
-    for n in range(1, 33):
+    for n in range(1, nmax + 1):
         a = _inputs(mode, n, 37, seed=1000 + n)
-        _, _, e_gpu, e_orc = _check(gpu, a, mode, f"n={n}")
+        _, _, e_gpu, e_orc = _check(gpu, a, mode, f"n={n} {layout}", lf)
         rows.append((n, e_gpu.max(), e_orc.max()))
     print("\n".join(f"  n={n:2d} e_gpu={g:.2e} e_orc={o:.2e}" for n, g, o in rows))
 
@@ -58,6 +65,13 @@ def test_all_sizes(gpu, mode):
 def test_batch_tails_n32(gpu, batch):
     a = oracle.uniform_matrices(42, batch, 32)
     _check(gpu, a, "parallel", f"batch={batch}")
+
+
+@pytest.mark.parametrize("batch", [1, 15, 16, 17, 33, 1000])
+def test_batch_tails_quads(gpu, batch):
+    for n in (4, 7, 16):
+        a = oracle.uniform_matrices(43, batch, n)
+        _check(gpu, a, "parallel", f"batch={batch} n={n} quad4", 0x20)
 
 
 def test_config1_n32_1000(gpu):
@@ -78,11 +92,14 @@ def test_ties_use_exact_rule(gpu):
         a = rng.randint(0, 10, size=(64, n, n)).astype(np.float32)
         a += np.float32(0.5) * (np.arange(n) == np.arange(n)[:, None])  # keep most of them invertible
         for mode in ("parallel", "serial", "tree"):
-            flags, rule = MODES[mode]
-            print(f"[case] ties n={n} mode={mode}", flush=True)
-            _, piv = gpu.invert(a, flags)
-            po = oracle.batch_pivots(a, rule)
-            assert (piv == po).all(), f"ties n={n} {mode}: pivots differ"
+            for layout, (lf, nmax) in LAYOUTS.items():
+                if n > nmax:
+                    continue
+                flags, rule = MODES[mode]
+                print(f"[case] ties n={n} mode={mode} {layout}", flush=True)
+                _, piv = gpu.invert(a, flags | lf)
+                po = oracle.batch_pivots(a, rule)
+                assert (piv == po).all(), f"ties n={n} {mode} {layout}: pivots differ"
 
 
 def test_nan_and_zero_pivot_rules(gpu):
