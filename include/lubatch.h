@@ -83,6 +83,47 @@ int lu_batched_inverse_async(float* A, int* piv, int n, int batch, int flags, vo
 /* 64-bit batch form of lu_batched_inverse_async. */
 int lu_batched_inverse_async64(float* A, int* piv, int n, int64_t batch, int flags, void* stream);
 
+/* ---- ragged (mixed-n) batches -----------------------------------------
+ * Matrix i has size ns[i] (1..32) and starts at float offset offsets[i] of
+ * A (row-major, contiguous, inverted in place); its pivot vector goes to
+ * piv + piv_offsets[i] (piv may be NULL).  All five arrays are device
+ * memory.  The batch is planned and inverted on the device by one
+ * persistent kernel (size classes largest first, per-class lane layout and
+ * matrices per work item from the tuning table); results equal a plain launch
+ * of each matrix's n.  Matrices with n outside 1..32 are skipped and counted
+ * (lu_ragged_invalid_count).  Replaces, for mixed sizes, one reference build
+ * and launch per MATRIXSIZE (parallel_pivot/run.py:198-234). */
+typedef struct lub_ragged_plan lub_ragged_plan;
+
+/* Workspace for batches of up to max_batch matrices (device memory of 16
+ * bytes per matrix); a plan may be used by one stream at a time. */
+int lu_ragged_plan_create(lub_ragged_plan** plan, int64_t max_batch);
+int lu_ragged_plan_destroy(lub_ragged_plan* plan);
+
+/* Enqueue the planning kernels and the persistent inversion kernel on
+ * `stream`.  Allocates nothing and never synchronises, so the call can be
+ * captured into a hipGraph and replayed. */
+int lu_batched_inverse_ragged_plan(lub_ragged_plan* plan, float* A, const int64_t* offsets, const int* ns, int* piv,
+                                   const int64_t* piv_offsets, int64_t batch, int flags, void* stream);
+
+/* Convenience form with a library-owned plan per host thread (created or
+ * grown on demand, which synchronises the device once).  Calls from one
+ * thread share that plan: issue them on one stream. */
+int lu_batched_inverse_ragged(float* A, const int64_t* offsets, const int* ns, int* piv, const int64_t* piv_offsets,
+                              int batch, int flags, void* stream);
+
+/* Matrices of the last enqueued batch whose n was outside 1..32 (waits for it). */
+int lu_ragged_invalid_count(const lub_ragged_plan* plan, int* out);
+
+/* ---- tuning ------------------------------------------------------------
+ * Per-n launch choices used when flags carry LUB_LAYOUT_AUTO: layout[n]
+ * (0 = 16-lane rows, 1 = quads; quads apply to n <= 16 in the plain kernel,
+ * n <= 12 in the ragged one) and grab[n] (ragged: wave tiles per work item,
+ * >= 1, used up to 16 matrices per item).  Arrays of 33 entries indexed by n
+ * (entry 0 unused). */
+int lu_tuning_set(const signed char* layout, const signed char* grab);
+int lu_tuning_get(signed char* layout, signed char* grab);
+
 /* Device-side inversion check, the reference's verifyInv (parallel_pivot/
  * verify.hpp:50-103) on the GPU.  For each of `batch` matrices, A (the
  * original matrices) times X (their computed inverses) is formed in FP32 in

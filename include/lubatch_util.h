@@ -43,6 +43,13 @@ int   lubx_event_sync(void* ev);
 /* Milliseconds between two recorded events (waits for `stop`); < 0 on error. */
 float lubx_event_elapsed_ms(void* start, void* stop);
 
+/* hipGraph capture of work enqueued on `stream` between begin and end; the
+ * instantiated graph replays it with one launch. */
+int   lubx_graph_capture_begin(void* stream);
+int   lubx_graph_capture_end(void* stream, void** graph_exec);
+int   lubx_graph_launch(void* graph_exec, void* stream);
+int   lubx_graph_destroy(void* graph_exec);
+
 /* Plain streaming copy kernel (dwordx4, grid-stride), used to calibrate the
  * FETCH_SIZE / WRITE_SIZE counters against a known byte count. */
 int   lubx_copy_f32(float* dst, const float* src, size_t count, void* stream);

@@ -22,6 +22,13 @@ ABI = {
     "lu_batched_inverse_async": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "lu_batched_inverse_async64": (_i, [_vp, _vp, _i, _i64, _i, _vp]),
     "lu_batched_residual": (_i, [_vp, _vp, _i, _i64, ctypes.c_float, _vp, _vp, _vp]),
+    "lu_ragged_plan_create": (_i, [ctypes.POINTER(_vp), _i64]),
+    "lu_ragged_plan_destroy": (_i, [_vp]),
+    "lu_batched_inverse_ragged_plan": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i, _vp]),
+    "lu_batched_inverse_ragged": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "lu_ragged_invalid_count": (_i, [_vp, ctypes.POINTER(_i)]),
+    "lu_tuning_set": (_i, [ctypes.c_char_p, ctypes.c_char_p]),
+    "lu_tuning_get": (_i, [ctypes.c_char_p, ctypes.c_char_p]),
     "lu_status_string": (ctypes.c_char_p, [_i]),
     "lu_last_error": (ctypes.c_char_p, []),
     "lu_version": (ctypes.c_char_p, []),
@@ -50,6 +57,10 @@ UTIL = {
     "lubx_event_sync": (_i, [_vp]),
     "lubx_event_elapsed_ms": (ctypes.c_float, [_vp, _vp]),
     "lubx_copy_f32": (_i, [_vp, _vp, _sz, _vp]),
+    "lubx_graph_capture_begin": (_i, [_vp]),
+    "lubx_graph_capture_end": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "lubx_graph_launch": (_i, [_vp, _vp]),
+    "lubx_graph_destroy": (_i, [_vp]),
 }
 
 _lib = None

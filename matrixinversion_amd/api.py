@@ -10,6 +10,8 @@ exceptions instead of exiting.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from ._native import check, lib
@@ -94,6 +96,120 @@ def invert(a: np.ndarray, flags: int = PIVOT_PARALLEL, return_pivots: bool = Tru
         A.close()
         if P is not None:
             P.close()
+
+
+class RaggedPlan:
+    """Device workspace for ragged batches of up to `max_batch` matrices
+    (lu_ragged_plan_create).  Enqueueing on a plan allocates nothing and never
+    synchronises, so it can be captured in a Graph."""
+
+    def __init__(self, max_batch: int):
+        h = ctypes.c_void_p()
+        check(lib().lu_ragged_plan_create(ctypes.byref(h), int(max_batch)), "lu_ragged_plan_create")
+        self.handle = h.value
+        self.max_batch = int(max_batch)
+
+    def invalid_count(self) -> int:
+        out = ctypes.c_int()
+        check(lib().lu_ragged_invalid_count(self.handle, ctypes.byref(out)), "lu_ragged_invalid_count")
+        return out.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().lu_ragged_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def lu_batched_inverse_ragged(A: DeviceBuffer, offsets: DeviceBuffer, ns: DeviceBuffer, batch: int,
+                              flags: int = PIVOT_PARALLEL, piv: DeviceBuffer | None = None,
+                              piv_offsets: DeviceBuffer | None = None, plan: RaggedPlan | None = None,
+                              stream: Stream | None = None) -> None:
+    """Invert a packed mixed-n batch in place: matrix i has size ns[i] and
+    starts at float offset offsets[i] (int64) of A; pivots go to
+    piv + piv_offsets[i].  Asynchronous on `stream` (default stream if None).
+    With a plan the call is lu_batched_inverse_ragged_plan, else the
+    per-thread-plan form."""
+    if batch > offsets.nbytes // 8 or batch > ns.nbytes // 4:
+        raise ValueError("offsets / ns hold fewer than batch entries")
+    if (piv is None) != (piv_offsets is None):
+        raise ValueError("piv and piv_offsets go together")
+    pp = piv.ptr if piv is not None else None
+    po = piv_offsets.ptr if piv_offsets is not None else None
+    sh = stream.handle if stream is not None else None
+    if plan is not None:
+        check(lib().lu_batched_inverse_ragged_plan(plan.handle, A.ptr, offsets.ptr, ns.ptr, pp, po, batch, flags, sh),
+              "lu_batched_inverse_ragged_plan")
+    else:
+        check(lib().lu_batched_inverse_ragged(A.ptr, offsets.ptr, ns.ptr, pp, po, batch, flags, sh),
+              "lu_batched_inverse_ragged")
+
+
+def pack_ragged(mats):
+    """Host packing of a list of square FP32 matrices: (flat A, offsets int64,
+    ns int32, pivot offsets int64)."""
+    ns = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    sizes = ns.astype(np.int64) ** 2
+    offsets = np.zeros(len(mats), dtype=np.int64)
+    if len(mats) > 1:
+        offsets[1:] = np.cumsum(sizes)[:-1]
+    poffs = np.zeros(len(mats), dtype=np.int64)
+    if len(mats) > 1:
+        poffs[1:] = np.cumsum(ns.astype(np.int64))[:-1]
+    flat = (np.concatenate([np.ascontiguousarray(m, dtype=np.float32).ravel() for m in mats])
+            if mats else np.zeros(0, np.float32))
+    return flat, offsets, ns, poffs
+
+
+def invert_ragged(mats, flags: int = PIVOT_PARALLEL):
+    """Host convenience for a list of square matrices of mixed n: returns
+    (inverses, pivots) as lists."""
+    flat, offsets, ns, poffs = pack_ragged(mats)
+    bufs = [DeviceBuffer.from_array(x) for x in (flat, offsets, ns)]
+    P = DeviceBuffer(max(int(ns.sum()) * 4, 4))
+    PO = DeviceBuffer.from_array(poffs)
+    try:
+        A, O, NS = bufs
+        lu_batched_inverse_ragged(A, O, NS, len(mats), flags, P, PO)
+        from .device import synchronize
+        synchronize()
+        x = A.download(flat.shape, np.float32)
+        p = P.download((int(ns.sum()),), np.int32)
+    finally:
+        for b in bufs + [P, PO]:
+            b.close()
+    inv = [x[o:o + n * n].reshape(n, n) for o, n in zip(offsets, ns)]
+    pv = [p[o:o + n] for o, n in zip(poffs, ns)]
+    return inv, pv
+
+
+def tuning_get():
+    """The per-n (layout, grab) table used with LAYOUT_AUTO (33 entries each)."""
+    layout = ctypes.create_string_buffer(33)
+    grab = ctypes.create_string_buffer(33)
+    check(lib().lu_tuning_get(layout, grab), "lu_tuning_get")
+    return (np.frombuffer(layout.raw, dtype=np.int8).copy(), np.frombuffer(grab.raw, dtype=np.int8).copy())
+
+
+def tuning_set(layout=None, grab=None) -> None:
+    """Replace the per-n table (None keeps the current column)."""
+    cur_l, cur_g = tuning_get()
+    lay = np.asarray(cur_l if layout is None else layout, dtype=np.int8)
+    grb = np.asarray(cur_g if grab is None else grab, dtype=np.int8)
+    if lay.shape != (33,) or grb.shape != (33,):
+        raise ValueError("tables hold 33 entries (index n, entry 0 unused)")
+    check(lib().lu_tuning_set(lay.tobytes(), grb.tobytes()), "lu_tuning_set")
 
 
 def version() -> str:

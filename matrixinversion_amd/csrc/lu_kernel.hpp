@@ -509,19 +509,17 @@ __device__ __forceinline__ void tile_front(ColBlock<N, L>& w, LaneInts<N, L>& pr
   wave_sync();
 }
 
-// Phases 6-8: (P*A)^-1 in registers, column q written to column perm[q] of the
-// LDS tile, the tile and the pivot vectors stored to HBM.
+// Phases 6-7: (P*A)^-1 in registers, column q written to column perm[q] of
+// the matrix's LDS region, which then holds A^-1 (row-major, stride LD).
 template <int N, int L>
-__device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L>& prm,
-                                          float* __restrict__ A, int* __restrict__ piv, int64_t m0, int valid,
-                                          bool vec, float* lds, int lane) {
+__device__ __forceinline__ void tile_solve(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* lds, int lane) {
   using G = Geo<N, L>;
   const int l = lane % L;
   const int g = lane / L;
   float* const mat = lds + g * G::MS;
 
   // 6. (P*A)^-1 = S * diag(cs), column q in lane q
-  float cs[G::R];
+  LaneFloats<N, L> cs;
   gauss_jordan<N, L>(w, cs, l);
 
   // 7. column q of (P*A)^-1 is column perm[q] of A^-1
@@ -534,8 +532,17 @@ __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L
     }
   }
   wave_sync();
+}
 
-  // 8. store the inverses and the pivot vectors
+// Phase 8 for a contiguous tile: the inverses and the pivot vectors to HBM.
+template <int N, int L>
+__device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* __restrict__ A,
+                                          int* __restrict__ piv, int64_t m0, int valid, bool vec, float* lds,
+                                          int lane) {
+  using G = Geo<N, L>;
+  const int l = lane % L;
+  const int g = lane / L;
+  tile_solve<N, L>(w, prm, lds, lane);
   tile_store<N, L>(A + m0 * (N * N), lds, valid, vec, lane);
   if (piv != nullptr && g < valid) {
 #pragma unroll

@@ -1,0 +1,394 @@
+// lu_ragged.hpp -- mixed-n batches in one inversion launch.
+//
+// The reference only knows batches of one n (compile-time MATRIXSIZE,
+// parallel_pivot/luBatchedInplace.cu:4); a mixed batch there means one build
+// and one launch per n.  Here a packed ragged batch -- matrix i of size
+// ns[i] at float offset offsets[i], pivots at piv_offsets[i] -- is inverted by
+// a short device-side plan and one persistent inversion kernel, all enqueued
+// on one stream without host synchronisation or allocation (so the sequence
+// can be captured once in a hipGraph and replayed):
+//
+//   clear     zero the counters of the workspace (a kernel, not a memset:
+//             memset nodes are kept out of captured graphs)
+//   hist      matrices per n (block-private LDS histograms, one global
+//             atomic per class per block)
+//   scan      the class table: size classes ordered largest n first; per
+//             class its first slot, its first descriptor, its lane layout and
+//             the matrices per descriptor (MPW(n) * grab(n) <= 16: MPW is 4
+//             with 16-lane rows, 16 with quads)
+//   scatter   each matrix's offset and pivot offset into its class's slot
+//             range (ranks from LDS atomics, one global atomic per class per
+//             block), so a descriptor's matrices sit in consecutive slots
+//   invert    persistent waves; wave w takes descriptors w, w + grid, ...
+//             (classes largest first, so every wave sees the same mix and the
+//             tail is made of the cheapest tiles).  A descriptor is located
+//             from the class table held in lanes 0..32 (ballot + readlane, no
+//             memory access), and the slot offsets of the next descriptor are
+//             loaded into lanes 0..15 while the current one is worked on, so
+//             no metadata load sits between a tile and its data loads.  The
+//             tile routines are the plain kernel's, with per-matrix gather
+//             loads and scatter stores.
+//
+// A matrix's result does not depend on which tile or slot it lands in: every
+// tile routine treats its matrices independently (the serial fallback of the
+// parallel pivot search reproduces the same first-max pivots), so results
+// are identical to a plain launch of the same n.
+#pragma once
+
+#include "lu_kernel.hpp"
+
+namespace lub {
+
+struct RaggedTuning {
+  int8_t layout[33];  // per n: 0 = 16-lane rows, 1 = quads (n <= 12 only in the ragged kernel)
+  int8_t grab[33];    // per n: wave tiles per descriptor (>= 1; at most 64 matrices per descriptor)
+};
+
+// Workspace: an int32 header, then two int64 slot arrays (offsets and pivot
+// offsets in class order).
+struct RaggedWs {
+  static constexpr int COUNTS = 0;        // [n] matrices of size n
+  static constexpr int CURSOR = 64;       // [n] slots handed out (scatter)
+  static constexpr int CLASS_FIRST = 128; // [n] first slot of class n
+  static constexpr int DESC_FIRST = 192;  // [n] first descriptor of class n; [0] descriptors in all
+  static constexpr int META = 256;        // [2] matrices with n outside 1..32
+  static constexpr int LPER = 320;        // [n] layout | matrices per descriptor << 8
+  static constexpr int HEADER = 384;
+  static int64_t slots_offset() { return HEADER * 4; }  // bytes, 16-aligned
+  static int64_t bytes(int64_t cap) { return slots_offset() + 2 * 8 * cap; }
+};
+
+constexpr int kRaggedQuadMaxN = 12;
+constexpr int kRaggedMaxPer = 64;  // matrices per descriptor: one slot per lane
+
+__host__ __device__ constexpr int ragged_mpw(int n, int layout) { return (layout == 1 && n <= kRaggedQuadMaxN) ? 16 : 4; }
+
+// ---------------------------------------------------------- inversion
+
+template <int A_, int B_>
+constexpr int cmax() { return A_ > B_ ? A_ : B_; }
+
+constexpr int kRaggedLdsFloats = cmax<Geo<32, 16>::LDS_FLOATS, Geo<kRaggedQuadMaxN, 4>::LDS_FLOATS>();
+
+// Float4 of a ragged batch: matrices start at any float offset, so global
+// accesses are only 4-byte aligned (gfx950 executes dword-aligned dwordx4).
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// Slot offsets of one descriptor, one slot per lane (lanes 0..15; lanes past
+// the descriptor's count repeat slot 0): matrix offset and pivot offset.
+struct RaggedSlots {
+  uint32_t so_lo, so_hi, po_lo, po_hi;
+};
+
+// Offset of slot `s` (wave-uniform) as a scalar.
+__device__ __forceinline__ int64_t slot_uniform(uint32_t lo, uint32_t hi, int s) {
+  const uint32_t l = __builtin_amdgcn_readlane(lo, s);
+  const uint32_t h = __builtin_amdgcn_readlane(hi, s);
+  return static_cast<int64_t>((uint64_t(h) << 32) | l);
+}
+
+// Offset of slot `s` (per lane); every lane of the wave must take part.
+__device__ __forceinline__ int64_t slot_lane(uint32_t lo, uint32_t hi, int s) {
+  const uint32_t l = __builtin_amdgcn_ds_bpermute(s * 4, static_cast<int>(lo));
+  const uint32_t h = __builtin_amdgcn_ds_bpermute(s * 4, static_cast<int>(hi));
+  return static_cast<int64_t>((uint64_t(h) << 32) | l);
+}
+
+// Registers of one tile's gather: for n >= 6 MPW matrices of K4 float4
+// chunks per lane (+ one trailing float when n*n % 4 != 0), else ROUNDS
+// floats per lane.
+template <int N, int L>
+struct RaggedRegs {
+  static constexpr int NN = N * N;
+  static constexpr bool WHOLE = NN > kWave / 2;
+  static constexpr int C4 = NN / 4, REM = NN % 4, K4 = (C4 + kWave - 1) / kWave;
+  static constexpr int PER = WHOLE ? 1 : kWave / NN;
+  static constexpr int ROUNDS = WHOLE ? 1 : (Geo<N, L>::MPW + PER - 1) / PER;
+  f32x4u v[WHOLE ? Geo<N, L>::MPW : 1][K4];
+  float r[WHOLE ? Geo<N, L>::MPW : ROUNDS];
+};
+
+// Gather loads of up to MPW matrices of one n (slots base .. base+valid-1 of
+// `sl`) into registers; slots past `valid` load the first matrix again, so
+// every load is unconditional and all of them are in flight together.  For
+// n >= 6 the whole wave reads one matrix at a time, 16 bytes per lane (1 KB
+// contiguous per load instruction) plus n*n % 4 trailing floats; for smaller
+// n one dword load instruction covers 64 / (n*n) matrices.
+template <int N, int L>
+__device__ __forceinline__ void ragged_load(const float* __restrict__ A, const RaggedSlots& sl, int base, int valid,
+                                            RaggedRegs<N, L>& x, int lane) {
+  using G = Geo<N, L>;
+  using X = RaggedRegs<N, L>;
+  if constexpr (X::WHOLE) {
+#pragma unroll
+    for (int g = 0; g < G::MPW; ++g) {
+      const float* src = A + slot_uniform(sl.so_lo, sl.so_hi, base + (g < valid ? g : 0));
+#pragma unroll
+      for (int k = 0; k < X::K4; ++k) {
+        const int c = lane + kWave * k;
+        if (X::C4 % kWave == 0 || c < X::C4)
+          x.v[g][k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u*>(src) + c);
+      }
+      if constexpr (X::REM != 0) x.r[g] = __builtin_nontemporal_load(src + 4 * X::C4 + (lane < X::REM ? lane : 0));
+    }
+  } else {
+    const int slot = lane / X::NN, e = lane - slot * X::NN;
+#pragma unroll
+    for (int r = 0; r < X::ROUNDS; ++r) {
+      const int g = r * X::PER + slot;
+      const int64_t off = slot_lane(sl.so_lo, sl.so_hi, base + ((slot < X::PER && g < valid) ? g : 0));
+      x.r[r] = __builtin_nontemporal_load(A + off + e);
+    }
+  }
+}
+
+// Registers -> the tile's LDS layout; slots past `valid` get the identity.
+template <int N, int L>
+__device__ __forceinline__ void ragged_stage(const RaggedRegs<N, L>& x, int valid, float* lds, int lane) {
+  using G = Geo<N, L>;
+  using X = RaggedRegs<N, L>;
+  if constexpr (X::WHOLE) {
+#pragma unroll
+    for (int g = 0; g < G::MPW; ++g) {
+      float* const mat = lds + g * G::MS;
+      const bool live = g < valid;
+#pragma unroll
+      for (int k = 0; k < X::K4; ++k) {
+        const int c = lane + kWave * k;
+        if (X::C4 % kWave == 0 || c < X::C4) {
+          if constexpr (N % 4 == 0) {
+            // a chunk is 4 consecutive floats of one row; LD and MS keep it 16-byte aligned in LDS
+            const int i = (4 * c) / N, j = (4 * c) % N;
+            f32x4 w;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = live ? x.v[g][k][q] : (i == j + q ? 1.0f : 0.0f);
+            *reinterpret_cast<f32x4*>(mat + i * G::LD + j) = w;
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int e = 4 * c + q, i = e / N, j = e % N;
+              mat[i * G::LD + j] = live ? x.v[g][k][q] : (i == j ? 1.0f : 0.0f);
+            }
+          }
+        }
+      }
+      if constexpr (X::REM != 0) {
+        if (lane < X::REM) {
+          const int e = 4 * X::C4 + lane, i = e / N, j = e % N;
+          mat[i * G::LD + j] = live ? x.r[g] : (i == j ? 1.0f : 0.0f);
+        }
+      }
+    }
+  } else {
+    const int slot = lane / X::NN, e = lane - slot * X::NN;
+    const int i = e / N, j = e - i * N;
+#pragma unroll
+    for (int r = 0; r < X::ROUNDS; ++r) {
+      const int g = r * X::PER + slot;
+      if (slot < X::PER && g < G::MPW)
+        lds[g * G::MS + i * G::LD + j] = (g < valid) ? x.r[r] : (i == j ? 1.0f : 0.0f);
+    }
+  }
+}
+
+// The mirror of ragged_load + ragged_stage for the `valid` leading matrices of the tile.
+template <int N, int L>
+__device__ __forceinline__ void ragged_scatter_store(float* __restrict__ A, const RaggedSlots& sl, int base,
+                                                     int valid, const float* lds, int lane) {
+  using G = Geo<N, L>;
+  constexpr int NN = N * N;
+  if constexpr (NN > kWave / 2) {
+    constexpr int C4 = NN / 4, REM = NN % 4, K4 = (C4 + kWave - 1) / kWave;
+#pragma unroll
+    for (int g = 0; g < G::MPW; ++g) {
+      if (g < valid) {
+        const float* const mat = lds + g * G::MS;
+        float* dst = A + slot_uniform(sl.so_lo, sl.so_hi, base + g);
+#pragma unroll
+        for (int k = 0; k < K4; ++k) {
+          const int c = lane + kWave * k;
+          if (C4 % kWave == 0 || c < C4) {
+            f32x4u w;
+            if constexpr (N % 4 == 0) {
+              const int i = (4 * c) / N, j = (4 * c) % N;
+              const f32x4 t = *reinterpret_cast<const f32x4*>(mat + i * G::LD + j);
+              w = f32x4u{t[0], t[1], t[2], t[3]};
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const int e = 4 * c + q;
+                w[q] = mat[(e / N) * G::LD + e % N];
+              }
+            }
+            __builtin_nontemporal_store(w, reinterpret_cast<f32x4u*>(dst) + c);
+          }
+        }
+        if constexpr (REM != 0) {
+          if (lane < REM) {
+            const int e = 4 * C4 + lane;
+            __builtin_nontemporal_store(mat[(e / N) * G::LD + e % N], dst + e);
+          }
+        }
+      }
+    }
+  } else {
+    constexpr int PER = kWave / NN;
+    constexpr int ROUNDS = (G::MPW + PER - 1) / PER;
+    const int slot = lane / NN, e = lane - slot * NN;
+    const int i = e / N, j = e - i * N;
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+      const int g = r * PER + slot;
+      const int64_t off = slot_lane(sl.so_lo, sl.so_hi, base + ((slot < PER && g < valid) ? g : 0));
+      if (slot < PER && g < valid) __builtin_nontemporal_store(lds[g * G::MS + i * G::LD + j], A + off + e);
+    }
+  }
+}
+
+// One descriptor (count <= 64 matrices of one n) in tiles of MPW: gather,
+// the plain kernel's tile routines (factor, invert, unpermute into LDS),
+// scatter, pivots.  The loads of tile b+1 are issued before the stores of
+// tile b: the memory counter retires in order, so loads issued after the
+// stores would also wait for the stores to complete.
+template <int N, int L, int PIV>
+__device__ __forceinline__ void ragged_tile(float* __restrict__ A, int* __restrict__ piv, const RaggedSlots& sl,
+                                            int count, float* lds, int lane) {
+  using G = Geo<N, L>;
+  static_assert(G::LDS_FLOATS <= kRaggedLdsFloats, "tile exceeds the ragged kernel's LDS array");
+  const int l = lane % L;
+  const int g = lane / L;
+  RaggedRegs<N, L> x;
+  ragged_load<N, L>(A, sl, 0, min(G::MPW, count), x, lane);
+  for (int base = 0;;) {
+    const int valid = min(G::MPW, count - base);
+    ragged_stage<N, L>(x, valid, lds, lane);
+    wave_sync();
+    ColBlock<N, L> w;
+    LaneInts<N, L> prm;
+    tile_front<N, L, PIV>(w, prm, lds, lane);
+    tile_solve<N, L>(w, prm, lds, lane);
+    auto finish = [&](int b0, int nv) {
+      ragged_scatter_store<N, L>(A, sl, b0, nv, lds, lane);
+      if (piv != nullptr) {
+        const int64_t po = slot_lane(sl.po_lo, sl.po_hi, b0 + (g < nv ? g : 0));
+        if (g < nv) {
+#pragma unroll
+          for (int h = 0; h < G::R; ++h)
+            if (l + L * h < N) piv[po + l + L * h] = prm[h];
+        }
+      }
+      wave_sync();  // the next tile reuses the LDS region
+    };
+    // Two exits so that only freshly loaded registers reach the loop head:
+    // the staged ones are dead during the factorisation.
+    const int nb = base + G::MPW;
+    if (nb < count) {
+      ragged_load<N, L>(A, sl, nb, min(G::MPW, count - nb), x, lane);
+      finish(base, valid);
+      base = nb;
+    } else {
+      finish(base, valid);
+      break;
+    }
+  }
+}
+
+struct RaggedDesc {
+  int n;
+  int layout;  // 0: 16-lane rows, 1: quads
+  int first;   // first slot
+  int count;   // matrices (<= 16)
+};
+
+// Class table of the plan, lane c (1..32) holding class c.
+struct RaggedClasses {
+  int desc_first, class_first, count, lper;
+};
+
+__device__ __forceinline__ RaggedClasses ragged_load_classes(const int* __restrict__ hdr, int lane) {
+  RaggedClasses c;
+  const bool row = lane <= 32;
+  c.desc_first = row ? hdr[RaggedWs::DESC_FIRST + lane] : 0x7fffffff;
+  c.class_first = row ? hdr[RaggedWs::CLASS_FIRST + lane] : 0;
+  c.count = row ? hdr[RaggedWs::COUNTS + lane] : 0;
+  c.lper = row ? hdr[RaggedWs::LPER + lane] : 0;
+  return c;
+}
+
+// Descriptor t (< the descriptor total): its class is the smallest n whose
+// first descriptor is <= t (classes are laid out 32, 31, ..., 1 and an empty
+// class starts where the next one does).
+__device__ __forceinline__ RaggedDesc ragged_locate(const RaggedClasses& c, int t, int lane) {
+  const uint64_t m = __ballot(lane >= 1 && lane <= 32 && c.desc_first <= t);
+  const int n = __builtin_ctzll(m);
+  const int df = __builtin_amdgcn_readlane(c.desc_first, n);
+  const int cf = __builtin_amdgcn_readlane(c.class_first, n);
+  const int cnt = __builtin_amdgcn_readlane(c.count, n);
+  const int lp = __builtin_amdgcn_readlane(c.lper, n);
+  const int per = lp >> 8;
+  const int first = cf + (t - df) * per;
+  return RaggedDesc{n, lp & 0xff, first, min(per, cf + cnt - first)};
+}
+
+__device__ __forceinline__ RaggedSlots ragged_load_slots(const int64_t* __restrict__ soffs,
+                                                         const int64_t* __restrict__ spoffs, const RaggedDesc& d,
+                                                         int lane) {
+  RaggedSlots s{0, 0, 0, 0};
+  {
+    const int i = d.first + (lane < d.count ? lane : 0);
+    const uint64_t so = static_cast<uint64_t>(soffs[i]);
+    s.so_lo = static_cast<uint32_t>(so);
+    s.so_hi = static_cast<uint32_t>(so >> 32);
+    if (spoffs != nullptr) {
+      const uint64_t po = static_cast<uint64_t>(spoffs[i]);
+      s.po_lo = static_cast<uint32_t>(po);
+      s.po_hi = static_cast<uint32_t>(po >> 32);
+    }
+  }
+  return s;
+}
+
+template <int PIV, int N>
+__device__ __forceinline__ void ragged_dispatch(const RaggedDesc& d, const RaggedSlots& sl, float* __restrict__ A,
+                                                int* __restrict__ piv, float* lds, int lane) {
+  if constexpr (N <= 32) {
+    if (d.n == N) {
+      if constexpr (N <= kRaggedQuadMaxN) {
+        if (d.layout == 1) {
+          ragged_tile<N, 4, PIV>(A, piv, sl, d.count, lds, lane);
+          return;
+        }
+      }
+      ragged_tile<N, 16, PIV>(A, piv, sl, d.count, lds, lane);
+    } else {
+      ragged_dispatch<PIV, N + 1>(d, sl, A, piv, lds, lane);
+    }
+  }
+}
+
+// Workgroup w (one wave) takes descriptors w, w + grid, ... below the
+// descriptor total, so every wave reaches the exit.  With a grid of at
+// least the descriptor count each workgroup does one descriptor and the
+// hardware dispatcher starts a new wave whenever one ends; a resident-size
+// grid makes the waves persistent.
+template <int PIV>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void lu_ragged_kernel(
+    float* __restrict__ A, const int64_t* __restrict__ soffs, int* __restrict__ piv,
+    const int64_t* __restrict__ spoffs, const int* __restrict__ hdr) {
+  __shared__ __attribute__((aligned(16))) float lds[kRaggedLdsFloats];
+  const int lane = threadIdx.x;
+  const RaggedClasses cls = ragged_load_classes(hdr, lane);
+  const int total = __builtin_amdgcn_readlane(cls.desc_first, 0);
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const RaggedDesc d = ragged_locate(cls, t, lane);
+    const RaggedSlots sl = ragged_load_slots(soffs, spoffs, d, lane);
+    // Opaque per descriptor: keeps the lane-derived addresses of all the
+    // (n, layout) variants from being hoisted out of the loop, where they
+    // would all stay live at once.
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    ragged_dispatch<PIV, 1>(d, sl, A, piv, lds, ln);
+  }
+}
+
+}  // namespace lub

@@ -35,9 +35,26 @@ const lub::KernelTable& table() {
     lub_register_21_24(&g_table);
     lub_register_25_28(&g_table);
     lub_register_29_32(&g_table);
+    lub_register_ragged_0(&g_table);
+    lub_register_ragged_1(&g_table);
+    lub_register_ragged_2(&g_table);
+    lub_register_ragged_3(&g_table);
   });
   return g_table;
 }
+
+// Per-n launch choices.  Defaults from MI355X measurements at 1M matrices
+// (quads for n <= 12; one wave tile per ragged queue descriptor), replaced
+// as a whole by lu_tuning_set (the autotuner's table).
+std::mutex g_tuning_mutex;
+lub::Tuning g_tuning = [] {
+  lub::Tuning t;
+  for (int n = 0; n <= lub::kMaxN; ++n) {
+    t.layout[n] = n <= 12 ? 1 : 0;
+    t.grab[n] = n <= 12 ? 1 : 2;
+  }
+  return t;
+}();
 
 // Workgroups are one wave (64 lanes) and own a tile of 4 matrices at a time;
 // the kernels stride over tiles.  Two launch shapes:
@@ -91,11 +108,14 @@ int resident_workgroups(const void* fn, int n, int pv, int layout, int cap_per_c
 }
 
 // Lanes per matrix for each n: 4-lane quads (16 matrices per wave) where the
-// 16-lane row would leave most lanes idle, 16-lane rows otherwise.  Measured
-// on MI355X at 1M matrices: quads are 3x faster at n <= 4, 1.5x at n = 8,
-// even at n = 12, slower at n = 16.  LUB_LANES=4|16 forces one layout (where
-// it exists).
-int default_layout(int n) { return n <= 12 ? 1 : 0; }
+// 16-lane row would leave most lanes idle, 16-lane rows otherwise (tuning
+// table; measured on MI355X at 1M matrices: quads are 3x faster at n <= 4,
+// 1.5x at n = 8, even at n = 12, slower at n = 16).  LUB_LANES=4|16 forces
+// one layout (where it exists).
+int default_layout(int n) {
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  return (g_tuning.layout[n] == 1 && n <= lub::kLayoutMaxN[1]) ? 1 : 0;
+}
 
 int choose_layout(int n, int flags) {
   switch (flags & LUB_LAYOUT_MASK) {
@@ -123,7 +143,38 @@ int validate(const float* A, const int* piv, int n, int64_t batch, int flags) {
 
 }  // namespace
 
+namespace lub {
+Tuning current_tuning() {
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  return g_tuning;
+}
+const KernelTable& kernel_table() { return table(); }
+}  // namespace lub
+
 extern "C" {
+
+int lu_tuning_set(const signed char* layout, const signed char* grab) {
+  if (layout == nullptr || grab == nullptr) return LUB_ENULL;
+  lub::Tuning t;
+  for (int n = 0; n <= lub::kMaxN; ++n) {
+    if (layout[n] < 0 || layout[n] > 1 || grab[n] < 1) return LUB_EINVAL_ARG;
+    t.layout[n] = static_cast<int8_t>(layout[n]);
+    t.grab[n] = static_cast<int8_t>(grab[n]);
+  }
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  g_tuning = t;
+  return LUB_OK;
+}
+
+int lu_tuning_get(signed char* layout, signed char* grab) {
+  if (layout == nullptr || grab == nullptr) return LUB_ENULL;
+  lub::Tuning t = lub::current_tuning();
+  for (int n = 0; n <= lub::kMaxN; ++n) {
+    layout[n] = t.layout[n];
+    grab[n] = t.grab[n];
+  }
+  return LUB_OK;
+}
 
 int lu_batched_inverse_async64(float* A, int* piv, int n, int64_t batch, int flags, void* stream) {
   int rc = validate(A, piv, n, batch, flags);

@@ -17,7 +17,17 @@ constexpr int kLayoutMaxN[kNumLayout] = {32, 16};
 // its range of N and registers the host stubs here).
 struct KernelTable {
   const void* plain[kMaxN + 1][kNumPivot][kNumLayout];
+  const void* ragged[kNumPivot];
 };
+
+// Per-n launch choices shared by the plain and the ragged paths (set by the
+// autotuner through lu_tuning_set, defaults in lub_api.hip).
+struct Tuning {
+  int8_t layout[kMaxN + 1];  // 0: 16-lane rows, 1: quads
+  int8_t grab[kMaxN + 1];    // ragged: wave tiles per queue descriptor
+};
+Tuning current_tuning();
+const KernelTable& kernel_table();
 
 }  // namespace lub
 
@@ -29,3 +39,7 @@ LUB_REGISTER_DECL(17, 20);
 LUB_REGISTER_DECL(21, 24);
 LUB_REGISTER_DECL(25, 28);
 LUB_REGISTER_DECL(29, 32);
+extern "C" void lub_register_ragged_0(lub::KernelTable* t);
+extern "C" void lub_register_ragged_1(lub::KernelTable* t);
+extern "C" void lub_register_ragged_2(lub::KernelTable* t);
+extern "C" void lub_register_ragged_3(lub::KernelTable* t);

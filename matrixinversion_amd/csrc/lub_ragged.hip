@@ -1,0 +1,301 @@
+// lub_ragged.hip -- C ABI of the ragged (mixed-n) path (include/lubatch.h).
+//
+// A plan owns the workspace of the device-side planning kernels; enqueueing
+// a batch on a plan allocates nothing and synchronises nothing, so the whole
+// sequence (clear, hist, scan, scatter, invert) can be captured in a hipGraph
+// once and replayed.  lu_batched_inverse_ragged keeps one plan per host
+// thread and grows it when a larger batch arrives.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "lu_ragged.hpp"
+#include "lub_internal.h"
+#include "lubatch.h"
+
+struct lub_ragged_plan {
+  int device = -1;
+  int64_t capacity = 0;  // matrices
+  char* ws = nullptr;    // lub::RaggedWs layout
+  int cus = 0;           // compute units of `device`
+  int resident[4] = {};  // resident waves of each pivot rule's kernel on `device`
+  int* header() const { return reinterpret_cast<int*>(ws); }
+  int64_t* soffs() const { return reinterpret_cast<int64_t*>(ws + lub::RaggedWs::slots_offset()); }
+  int64_t* spoffs() const { return soffs() + capacity; }
+};
+
+namespace lub {
+
+// Planning kernels (one translation unit: they are not templates).
+
+constexpr int kPlanThreads = 256;
+constexpr int kPlanItems = 16;  // matrices per thread: 4096 per block
+constexpr int64_t kPlanChunk = int64_t(kPlanThreads) * kPlanItems;
+
+__global__ void ragged_clear(int* ws) {
+  for (int i = threadIdx.x; i < RaggedWs::HEADER; i += blockDim.x) ws[i] = 0;
+}
+
+__global__ __launch_bounds__(kPlanThreads) void ragged_hist(const int* __restrict__ ns, int64_t batch, int* ws) {
+  __shared__ int hist[34];  // [0] matrices with n outside 1..32
+  if (threadIdx.x < 34) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t b0 = int64_t(blockIdx.x) * kPlanChunk;
+#pragma unroll
+  for (int k = 0; k < kPlanItems; ++k) {
+    const int64_t i = b0 + k * kPlanThreads + threadIdx.x;
+    if (i < batch) {
+      const int n = ns[i];
+      atomicAdd(&hist[(n >= 1 && n <= 32) ? n : 0], 1);
+    }
+  }
+  __syncthreads();
+  const int c = threadIdx.x;
+  if (c >= 1 && c <= 32 && hist[c] != 0) atomicAdd(&ws[RaggedWs::COUNTS + c], hist[c]);
+  if (c == 0 && hist[0] != 0) atomicAdd(&ws[RaggedWs::META + 2], hist[0]);
+}
+
+// The class table: classes 32, 31, ..., 1 in order; a descriptor holds
+// MPW(n) * grab(n) matrices, at most kRaggedMaxPer (one slot per lane), so
+// grab is clamped to 16 with 16-lane rows and 4 with quads.
+__global__ void ragged_scan(int* ws, RaggedTuning tune) {
+  if (threadIdx.x != 0) return;
+  int first = 0, tiles = 0;
+  for (int n = 32; n >= 1; --n) {
+    const int count = ws[RaggedWs::COUNTS + n];
+    const int layout = (tune.layout[n] == 1 && n <= kRaggedQuadMaxN) ? 1 : 0;
+    const int mpw = ragged_mpw(n, layout);
+    const int per = mpw * min(max(1, (int)tune.grab[n]), kRaggedMaxPer / mpw);
+    ws[RaggedWs::CLASS_FIRST + n] = first;
+    ws[RaggedWs::DESC_FIRST + n] = tiles;
+    ws[RaggedWs::LPER + n] = layout | (per << 8);
+    first += count;
+    tiles += (count + per - 1) / per;
+  }
+  ws[RaggedWs::DESC_FIRST + 0] = tiles;
+  ws[RaggedWs::CLASS_FIRST + 0] = first;
+  ws[RaggedWs::LPER + 0] = 0;
+}
+
+// Slot of each matrix = its class's first slot + the block's base in the
+// class (one global atomic per class per block) + its rank in the block
+// (LDS atomics).  The order within a class is therefore not fixed; results
+// do not depend on it.
+__global__ __launch_bounds__(kPlanThreads) void ragged_scatter(const int* __restrict__ ns,
+                                                               const int64_t* __restrict__ offs,
+                                                               const int64_t* __restrict__ poffs, int64_t batch,
+                                                               int* ws, int64_t* __restrict__ soffs,
+                                                               int64_t* __restrict__ spoffs) {
+  __shared__ int hist[33];
+  __shared__ int base[33];
+  if (threadIdx.x < 33) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t b0 = int64_t(blockIdx.x) * kPlanChunk;
+  int n_of[kPlanItems];
+  int rank[kPlanItems];
+#pragma unroll
+  for (int k = 0; k < kPlanItems; ++k) {
+    const int64_t i = b0 + k * kPlanThreads + threadIdx.x;
+    const int n = i < batch ? ns[i] : 0;
+    n_of[k] = (n >= 1 && n <= 32) ? n : 0;
+    rank[k] = n_of[k] ? atomicAdd(&hist[n_of[k]], 1) : 0;
+  }
+  __syncthreads();
+  const int c = threadIdx.x;
+  if (c >= 1 && c <= 32 && hist[c] != 0)
+    base[c] = ws[RaggedWs::CLASS_FIRST + c] + atomicAdd(&ws[RaggedWs::CURSOR + c], hist[c]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPlanItems; ++k) {
+    if (n_of[k] == 0) continue;
+    const int64_t i = b0 + k * kPlanThreads + threadIdx.x;
+    const int pos = base[n_of[k]] + rank[k];
+    soffs[pos] = offs[i];
+    if (poffs != nullptr) spoffs[pos] = poffs[i];
+  }
+}
+
+}  // namespace lub
+
+namespace {
+
+constexpr int64_t kMaxRaggedBatch = (int64_t(1) << 31) - 64;
+
+int resident_per_cu(const void* fn) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0) != hipSuccess) return 0;
+  return per_cu;
+}
+
+int plan_alloc(lub_ragged_plan* p, int64_t capacity) {
+  void* ws = nullptr;
+  if (hipMalloc(&ws, lub::RaggedWs::bytes(capacity)) != hipSuccess) return LUB_EHIP;
+  p->ws = static_cast<char*>(ws);
+  p->capacity = capacity;
+  return LUB_OK;
+}
+
+void plan_release(lub_ragged_plan* p) {
+  if (p->ws) (void)hipFree(p->ws);
+  p->ws = nullptr;
+  p->capacity = 0;
+}
+
+unsigned plan_blocks(int64_t batch) { return static_cast<unsigned>((batch + lub::kPlanChunk - 1) / lub::kPlanChunk); }
+
+// Matrices per descriptor of class n under tuning `rt` (the scan kernel
+// computes the same).
+int ragged_per(const lub::RaggedTuning& rt, int n) {
+  const int layout = (rt.layout[n] == 1 && n <= lub::kRaggedQuadMaxN) ? 1 : 0;
+  const int mpw = lub::ragged_mpw(n, layout);
+  return mpw * std::min(std::max(1, int(rt.grab[n])), lub::kRaggedMaxPer / mpw);
+}
+
+// Grid of the inversion kernel: by default at least the descriptor count
+// (ceil(batch / smallest matrices-per-descriptor) + 32 bounds it), so each
+// workgroup does one descriptor; LUB_RAGGED_WAVES_PER_CU=<k> launches k
+// persistent waves per compute unit instead.
+int64_t ragged_grid(const lub_ragged_plan* p, const lub::RaggedTuning& rt, int64_t batch) {
+  static const int per_cu = [] {
+    const char* e = std::getenv("LUB_RAGGED_WAVES_PER_CU");
+    return e != nullptr ? std::atoi(e) : 0;
+  }();
+  if (per_cu > 0) return int64_t(p->cus) * per_cu;
+  int min_per = lub::kRaggedMaxPer;
+  for (int n = 1; n <= 32; ++n) min_per = std::min(min_per, ragged_per(rt, n));
+  return std::min<int64_t>((batch + min_per - 1) / min_per + 32, int64_t(1) << 25);  // grid * 64 within 32 bits
+}
+
+struct ThreadPlan {
+  lub_ragged_plan* plan = nullptr;
+  ~ThreadPlan() {
+    if (plan) {
+      plan_release(plan);
+      delete plan;
+    }
+  }
+};
+thread_local ThreadPlan g_thread_plan;
+
+}  // namespace
+
+extern "C" {
+
+int lu_ragged_plan_create(lub_ragged_plan** plan, int64_t max_batch) {
+  if (plan == nullptr) return LUB_ENULL;
+  // matrix indices and descriptor starts are int32 on the device
+  if (max_batch < 0 || max_batch > kMaxRaggedBatch) return LUB_EINVAL_BATCH;
+  auto* p = new lub_ragged_plan;
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return LUB_EHIP;
+  }
+  // Occupancy is queried here, not at enqueue time, so that an enqueue under
+  // stream capture makes launches only.
+  const lub::KernelTable& kt = lub::kernel_table();
+  if (hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
+      p->cus <= 0) {
+    delete p;
+    return LUB_ENODEVICE;
+  }
+  for (int pv = 0; pv < 4; ++pv) {
+    p->resident[pv] = kt.ragged[pv] ? p->cus * resident_per_cu(kt.ragged[pv]) : 0;
+    if (p->resident[pv] <= 0) {
+      delete p;
+      return LUB_ENODEVICE;
+    }
+  }
+  int rc = plan_alloc(p, std::max<int64_t>(max_batch, 1));
+  if (rc != LUB_OK) {
+    delete p;
+    return rc;
+  }
+  *plan = p;
+  return LUB_OK;
+}
+
+int lu_ragged_plan_destroy(lub_ragged_plan* plan) {
+  if (plan == nullptr) return LUB_ENULL;
+  plan_release(plan);
+  delete plan;
+  return LUB_OK;
+}
+
+int lu_batched_inverse_ragged_plan(lub_ragged_plan* plan, float* A, const int64_t* offsets, const int* ns, int* piv,
+                                   const int64_t* piv_offsets, int64_t batch, int flags, void* stream) {
+  if (plan == nullptr) return LUB_ENULL;
+  if (batch < 0 || batch > kMaxRaggedBatch) return LUB_EINVAL_BATCH;
+  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || (flags & LUB_PIVOT_MASK) > LUB_PIVOT_TREE ||
+      (flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_MASK)
+    return LUB_EFLAGS;
+  if (batch == 0) return LUB_OK;
+  if (A == nullptr || offsets == nullptr || ns == nullptr) return LUB_ENULL;
+  if (piv != nullptr && piv_offsets == nullptr) return LUB_ENULL;
+  if (batch > plan->capacity) return LUB_EINVAL_ARG;
+  const int pv = flags & LUB_PIVOT_MASK;
+  const void* fn = lub::kernel_table().ragged[pv];
+  if (fn == nullptr) return LUB_ENODEVICE;
+
+  lub::Tuning tuning = lub::current_tuning();
+  lub::RaggedTuning rt;
+  for (int n = 0; n <= 32; ++n) {
+    int layout = tuning.layout[n];
+    if ((flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_ROW16) layout = 0;
+    if ((flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_QUAD4) layout = 1;
+    rt.layout[n] = static_cast<int8_t>(layout);
+    rt.grab[n] = tuning.grab[n];
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int* hdr = plan->header();
+  const unsigned blocks = plan_blocks(batch);
+  lub::ragged_clear<<<1, 256, 0, s>>>(hdr);
+  lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, hdr);
+  lub::ragged_scan<<<1, 64, 0, s>>>(hdr, rt);
+  lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, hdr,
+                                                            plan->soffs(), plan->spoffs());
+  int64_t* soffs = plan->soffs();
+  int64_t* spoffs = piv ? plan->spoffs() : nullptr;
+  const int* hdr_arg = hdr;
+  void* args[] = {&A, &soffs, &piv, &spoffs, &hdr_arg};
+  const int64_t grid = ragged_grid(plan, rt, batch);
+  hipError_t e = hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(64), args, 0, s);
+  if (e != hipSuccess) return LUB_EHIP;
+  return hipGetLastError() == hipSuccess ? LUB_OK : LUB_EHIP;
+}
+
+int lu_batched_inverse_ragged(float* A, const int64_t* offsets, const int* ns, int* piv, const int64_t* piv_offsets,
+                              int batch, int flags, void* stream) {
+  if (batch < 0) return LUB_EINVAL_BATCH;
+  if (batch == 0) return LUB_OK;
+  ThreadPlan& tp = g_thread_plan;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return LUB_EHIP;
+  if (tp.plan == nullptr || tp.plan->device != dev || tp.plan->capacity < batch) {
+    if (tp.plan != nullptr) {
+      // the previous plan may still be in use by work on some stream
+      if (hipDeviceSynchronize() != hipSuccess) return LUB_EHIP;
+      plan_release(tp.plan);
+      delete tp.plan;
+      tp.plan = nullptr;
+    }
+    lub_ragged_plan* p = nullptr;
+    int rc = lu_ragged_plan_create(&p, std::max<int64_t>(batch, 1024));
+    if (rc != LUB_OK) return rc;
+    tp.plan = p;
+  }
+  return lu_batched_inverse_ragged_plan(tp.plan, A, offsets, ns, piv, piv_offsets, batch, flags, stream);
+}
+
+int lu_ragged_invalid_count(const lub_ragged_plan* plan, int* out) {
+  if (plan == nullptr || out == nullptr) return LUB_ENULL;
+  return hipMemcpy(out, plan->header() + lub::RaggedWs::META + 2, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess
+             ? LUB_OK
+             : LUB_EHIP;
+}
+
+}  // extern "C"

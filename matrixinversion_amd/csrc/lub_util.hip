@@ -121,6 +121,28 @@ float lubx_event_elapsed_ms(void* start, void* stop) {
   return ms;
 }
 
+int lubx_graph_capture_begin(void* stream) {
+  return rc(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
+}
+
+int lubx_graph_capture_end(void* stream, void** graph_exec) {
+  if (graph_exec == nullptr) return LUB_ENULL;
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(static_cast<hipStream_t>(stream), &g) != hipSuccess) return LUB_EHIP;
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return LUB_EHIP;
+  *graph_exec = ex;
+  return LUB_OK;
+}
+
+int lubx_graph_launch(void* graph_exec, void* stream) {
+  return rc(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), static_cast<hipStream_t>(stream)));
+}
+
+int lubx_graph_destroy(void* graph_exec) { return rc(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec))); }
+
 int lubx_copy_f32(float* dst, const float* src, size_t count, void* stream) {
   if (count == 0) return LUB_OK;
   if (dst == nullptr || src == nullptr) return LUB_ENULL;

@@ -151,3 +151,34 @@ class Event:
 def copy_kernel(dst: DeviceBuffer, src: DeviceBuffer, count: int, stream: Stream | None = None) -> None:
     """Run the plain dwordx4 copy kernel over `count` floats."""
     check(lib().lubx_copy_f32(dst.ptr, src.ptr, count, stream.handle if stream else None), "copy kernel")
+
+
+class Graph:
+    """Work enqueued on `stream` between begin() and end(), replayed by launch()
+    (hipStreamBeginCapture / hipGraphInstantiate / hipGraphLaunch)."""
+
+    def __init__(self, stream: Stream):
+        self.stream = stream
+        self.handle = None
+
+    def begin(self) -> None:
+        check(lib().lubx_graph_capture_begin(self.stream.handle), "graph capture begin")
+
+    def end(self) -> None:
+        h = ctypes.c_void_p()
+        check(lib().lubx_graph_capture_end(self.stream.handle, ctypes.byref(h)), "graph capture end")
+        self.handle = h.value
+
+    def launch(self) -> None:
+        check(lib().lubx_graph_launch(self.handle, self.stream.handle), "graph launch")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().lubx_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -37,35 +37,10 @@ import numpy as np
 
 from . import api
 from .device import DeviceBuffer, Event, Stream, copy_kernel, device_info
+from .workloads import make_inputs
 
 HBM_PEAK_GBPS = 8000.0
 THREADS_PER_WORKGROUP = 64  # one wave owns a tile of 4 matrices
-
-
-def _mt19937_uniform(seed: int, count: int) -> np.ndarray:
-    """U[-1,1) FP32 from std::mt19937(seed): numpy's MT19937 with legacy seeding
-    is the same generator (init_genrand); the mapping is pinned, not
-    implementation-defined."""
-    bg = np.random.MT19937(0)
-    bg._legacy_seeding(seed)
-    out = np.empty(count, dtype=np.float32)
-    step = 1 << 24
-    for s in range(0, count, step):
-        raw = bg.random_raw(min(step, count - s)).astype(np.uint32)
-        out[s:s + raw.size] = (raw >> 8).astype(np.float32) * np.float32(2.0 ** -23) - np.float32(1.0)
-    return out
-
-
-def make_inputs(kind: str, n: int, batch: int, seed: int) -> np.ndarray:
-    if kind in ("random", "dominant"):
-        a = _mt19937_uniform(seed, batch * n * n).reshape(batch, n, n)
-        if kind == "dominant":
-            a[:, np.arange(n), np.arange(n)] += np.float32(n)
-        return a
-    if kind.startswith("template:"):
-        t = np.loadtxt(kind.split(":", 1)[1], dtype=np.float32).ravel()[: n * n].reshape(n, n)
-        return np.broadcast_to(t, (batch, n, n)).copy()
-    raise ValueError(f"unknown input kind {kind!r}")
 
 
 def _batch_label(batch: int) -> str:

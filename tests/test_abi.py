@@ -58,6 +58,63 @@ def test_host_side_validation_codes():
     assert lib.lu_version().decode().count(".") == 2
 
 
+def test_ragged_and_tuning_validation_codes():
+    import ctypes
+
+    import numpy as np
+
+    from matrixinversion_amd import _native, api
+
+    lib = _native.lib()
+    fake = 0x1000
+    h = ctypes.c_void_p()
+    assert lib.lu_ragged_plan_create(None, 16) == 3                          # LUB_ENULL
+    assert lib.lu_ragged_plan_create(ctypes.byref(h), -1) == 2               # LUB_EINVAL_BATCH
+    assert lib.lu_ragged_plan_create(ctypes.byref(h), 2**31) == 2            # int32 indices on the device
+    assert lib.lu_ragged_plan_destroy(None) == 3
+    assert lib.lu_batched_inverse_ragged_plan(None, fake, fake, fake, None, None, 4, 1, None) == 3
+    assert lib.lu_batched_inverse_ragged(fake, fake, fake, None, None, -1, 1, None) == 2
+    assert lib.lu_batched_inverse_ragged(None, None, None, None, None, 0, 1, None) == 0
+    assert lib.lu_batched_inverse_ragged(fake, fake, fake, None, None, 0, 0x31, None) == 0  # empty first
+    h2 = ctypes.c_void_p(0x2000)  # flags are checked before the plan is touched
+    assert lib.lu_batched_inverse_ragged_plan(h2, fake, fake, fake, None, None, 4, 0x31, None) == 4
+    assert lib.lu_batched_inverse_ragged_plan(h2, fake, fake, fake, None, None, 4, 0x40, None) == 4
+    assert lib.lu_batched_inverse_ragged_plan(h2, fake, fake, fake, None, None, -2, 1, None) == 2
+    assert lib.lu_ragged_invalid_count(None, None) == 3
+
+    layout, grab = api.tuning_get()
+    assert layout.shape == (33,) and grab.shape == (33,)
+    assert set(np.unique(layout)) <= {0, 1} and (grab >= 1).all()
+    bad = grab.copy()
+    bad[5] = 0
+    assert lib.lu_tuning_set(layout.tobytes(), bad.tobytes()) == 8         # LUB_EINVAL_ARG
+    bad = layout.copy()
+    bad[3] = 2
+    assert lib.lu_tuning_set(bad.tobytes(), grab.tobytes()) == 8
+    assert lib.lu_tuning_set(None, grab.tobytes()) == 3
+    try:
+        g2 = grab.copy()
+        g2[7] = 5
+        api.tuning_set(grab=g2)
+        assert api.tuning_get()[1][7] == 5
+    finally:
+        api.tuning_set(layout, grab)
+    assert (api.tuning_get()[1] == grab).all()
+
+
+def test_pack_ragged_host_layout():
+    import numpy as np
+
+    from matrixinversion_amd import api
+
+    mats = [np.full((n, n), n, np.float32) for n in (3, 1, 32, 2)]
+    flat, offs, ns, poffs = api.pack_ragged(mats)
+    assert ns.tolist() == [3, 1, 32, 2]
+    assert offs.tolist() == [0, 9, 10, 1034]
+    assert poffs.tolist() == [0, 3, 4, 36]
+    assert flat.size == 1038 and flat[9] == 1 and flat[1034] == 2
+
+
 def test_no_cpu_fallback_in_product():
     """The product path never reaches the oracle."""
     pkg = os.path.join(ROOT, "matrixinversion_amd")

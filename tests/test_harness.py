@@ -30,3 +30,27 @@ def test_sizes_and_labels():
     assert harness._batch_label(100000) == "100k"
     assert harness._batch_label(1000000) == "1M"
     assert harness._batch_label(1234) == "1234"
+
+
+def test_workload_generators_match_the_oracle():
+    import oracle
+    from matrixinversion_amd import workloads
+
+    assert workloads.mt19937_raw(42, 3).tolist() == [1608637542, 3421126067, 4083286876]
+    assert (workloads.mt19937_raw(7, 1000) == oracle.mt19937_raw(7, 1000)).all()
+    for n in (1, 5, 32):
+        assert (workloads.uniform_matrices(11, 3, n) == oracle.uniform_matrices(11, 3, n)).all()
+        assert (workloads.dominant_matrices(11, 3, n) == oracle.dominant_matrices(11, 3, n)).all()
+
+
+def test_ragged_batch_layout():
+    from matrixinversion_amd import workloads
+
+    flat, offs, ns, poffs = workloads.ragged_batch(42, 5000)
+    assert ns.min() == 2 and ns.max() == 32 and abs(ns.mean() - 17) < 0.5
+    sq = ns.astype(np.int64) ** 2
+    assert (offs[1:] == offs[:-1] + sq[:-1]).all() and offs[0] == 0
+    assert flat.size == sq.sum()
+    assert (poffs[1:] == poffs[:-1] + ns[:-1]).all()
+    assert (flat == workloads.mt19937_uniform(42, flat.size)).all()
+    assert workloads.ragged_bytes(ns) == int((8 * sq + 4 * ns + 20).sum())
