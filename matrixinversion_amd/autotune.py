@@ -1,0 +1,182 @@
+"""Per-n launch tuning for the plain and ragged entry points.
+
+The reference fixes its launch shape by hand per build (NUMTHREADS and the
+derived matrices per block, parallel_pivot/run.py:39-41 and
+luBatchedInplace.cu:8-11).  Here the shape is a runtime table, one entry per
+n, read by liblubatch whenever flags carry LUB_LAYOUT_AUTO:
+
+  layout[n]  0 = 16 lanes per matrix (4 matrices per wave), 1 = one 4-lane
+             quad per matrix (16 per wave; plain path n <= 16, ragged n <= 12)
+  grab[n]    ragged path: wave tiles per work item (up to 64 matrices)
+
+This module measures both on the current device and writes the table, which
+load_table() installs in a process (the library's built-in defaults are a
+table measured this way on MI355X):
+
+    python -m matrixinversion_amd.autotune --out tuning.json
+
+The layout is timed with the plain kernel (1 launch over `batch` matrices of
+one n, pivoted); grab is timed with the ragged kernel on a single-class
+batch of the same n, so each choice is made where it matters.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import api
+from .device import DeviceBuffer, Event, Stream, copy_kernel, device_info, synchronize
+from .workloads import uniform_matrices
+
+PLAIN_QUAD_MAX_N = 16
+RAGGED_QUAD_MAX_N = 12
+MAX_PER = 64
+
+
+def _time(fn, stream: Stream, reps: int, trials: int = 3) -> float:
+    """Median over `trials` of the mean ms per call of `fn` over `reps`
+    back-to-back calls on `stream`."""
+    fn()
+    stream.synchronize()
+    e0, e1 = Event(), Event()
+    out = []
+    for _ in range(trials):
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        out.append(e0.elapsed_ms(e1) / reps)
+    return float(np.median(out))
+
+
+def tune_layout(n: int, batch: int, stream: Stream, reps: int = 20) -> tuple[int, dict]:
+    """Plain path: 16-lane rows vs quads for one n (n <= 16)."""
+    a = uniform_matrices(1000 + n, batch, n)
+    src = DeviceBuffer.from_array(a)
+    work = DeviceBuffer(a.nbytes)
+    piv = DeviceBuffer(batch * n * 4)
+    times = {}
+    try:
+        for name, flag in (("row16", api.LAYOUT_ROW16), ("quad4", api.LAYOUT_QUAD4)):
+            if flag == api.LAYOUT_QUAD4 and n > PLAIN_QUAD_MAX_N:
+                continue
+
+            def run():
+                # inverting in place alternates A and A^-1; restore every call so
+                # each launch sees the original inputs (the copy is timed too, but
+                # equally for both layouts)
+                copy_kernel(work, src, batch * n * n, stream)
+                api.lu_batched_inverse(work, n, batch, api.PIVOT_PARALLEL | flag, piv, stream)
+
+            def copy_only():
+                copy_kernel(work, src, batch * n * n, stream)
+
+            times[name] = _time(run, stream, reps) - _time(copy_only, stream, reps)
+    finally:
+        for b in (src, work, piv):
+            b.close()
+    best = min(times, key=times.get)
+    return (1 if best == "quad4" else 0), times
+
+
+def tune_grab(n: int, layout: int, batch: int, stream: Stream, reps: int = 20) -> tuple[int, dict]:
+    """Ragged path: wave tiles per work item for one n under `layout`."""
+    mpw = 16 if (layout == 1 and n <= RAGGED_QUAD_MAX_N) else 4
+    a = uniform_matrices(2000 + n, batch, n).reshape(-1)
+    ns = np.full(batch, n, np.int32)
+    offs = np.arange(batch, dtype=np.int64) * (n * n)
+    poffs = np.arange(batch, dtype=np.int64) * n
+    src = DeviceBuffer.from_array(a)
+    work = DeviceBuffer(a.nbytes)
+    O, NS, PO = (DeviceBuffer.from_array(x) for x in (offs, ns, poffs))
+    P = DeviceBuffer(batch * n * 4)
+    plan = api.RaggedPlan(batch)
+    base_l, base_g = api.tuning_get()
+    times = {}
+    try:
+        for grab in (1, 2, 4, 8, 16):
+            if mpw * grab > MAX_PER:
+                continue
+            g = base_g.copy()
+            g[n] = grab
+            api.tuning_set(base_l, g)
+
+            def run():
+                copy_kernel(work, src, a.size, stream)
+                api.lu_batched_inverse_ragged(work, O, NS, batch, api.PIVOT_PARALLEL, P, PO, plan=plan,
+                                              stream=stream)
+
+            def copy_only():
+                copy_kernel(work, src, a.size, stream)
+
+            times[grab] = _time(run, stream, reps) - _time(copy_only, stream, reps)
+    finally:
+        api.tuning_set(base_l, base_g)
+        plan.close()
+        for b in (src, work, O, NS, PO, P):
+            b.close()
+    best = min(times, key=times.get)
+    return best, times
+
+
+def autotune(sizes=range(1, 33), plain_batch: int = 262144, ragged_batch: int = 131072, log=None) -> dict:
+    stream = Stream()
+    layout, grab = api.tuning_get()
+    layout = layout.copy()
+    grab = grab.copy()
+    detail = {}
+    for n in sizes:
+        t0 = time.perf_counter()
+        if n <= PLAIN_QUAD_MAX_N:
+            layout[n], lt = tune_layout(n, plain_batch, stream)
+        else:
+            layout[n], lt = 0, {}
+        api.tuning_set(layout, grab)
+        grab[n], gt = tune_grab(n, int(layout[n]), ragged_batch, stream)
+        api.tuning_set(layout, grab)
+        detail[n] = {"layout_ms": lt, "grab_ms": {str(k): v for k, v in gt.items()}}
+        if log:
+            log(f"[tune] n={n:2d} layout={'quad4' if layout[n] else 'row16'} grab={grab[n]} "
+                f"({time.perf_counter() - t0:.1f}s) {lt} {gt}")
+    stream.close()
+    info = device_info(0)
+    return {"device": info.name, "arch": info.arch, "layout": [int(x) for x in layout],
+            "grab": [int(x) for x in grab], "plain_batch": plain_batch, "ragged_batch": ragged_batch,
+            "detail": detail}
+
+
+def apply_table(table: dict) -> None:
+    """Install a table written by autotune() (33 entries per column)."""
+    api.tuning_set(np.array(table["layout"], np.int8), np.array(table["grab"], np.int8))
+
+
+def load_table(path: str) -> None:
+    with open(path) as f:
+        apply_table(json.load(f))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default="tuning.json")
+    ap.add_argument("--sizes", default="1-32")
+    ap.add_argument("--plain-batch", type=int, default=262144)
+    ap.add_argument("--ragged-batch", type=int, default=131072)
+    args = ap.parse_args(argv)
+    lo, _, hi = args.sizes.partition("-")
+    sizes = range(int(lo), int(hi or lo) + 1)
+    table = autotune(sizes, args.plain_batch, args.ragged_batch,
+                     log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    synchronize()
+    with open(args.out, "w") as f:
+        json.dump(table, f, indent=1)
+    print(json.dumps({"layout": table["layout"], "grab": table["grab"]}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -4,7 +4,7 @@
 // parallel_pivot/luBatchedInplace.cu:4); a mixed batch there means one build
 // and one launch per n.  Here a packed ragged batch -- matrix i of size
 // ns[i] at float offset offsets[i], pivots at piv_offsets[i] -- is inverted by
-// a short device-side plan and one persistent inversion kernel, all enqueued
+// a short device-side plan and one inversion kernel, all enqueued
 // on one stream without host synchronisation or allocation (so the sequence
 // can be captured once in a hipGraph and replayed):
 //
@@ -14,20 +14,19 @@
 //             atomic per class per block)
 //   scan      the class table: size classes ordered largest n first; per
 //             class its first slot, its first descriptor, its lane layout and
-//             the matrices per descriptor (MPW(n) * grab(n) <= 16: MPW is 4
+//             the matrices per descriptor (MPW(n) * grab(n) <= 64: MPW is 4
 //             with 16-lane rows, 16 with quads)
 //   scatter   each matrix's offset and pivot offset into its class's slot
 //             range (ranks from LDS atomics, one global atomic per class per
 //             block), so a descriptor's matrices sit in consecutive slots
-//   invert    persistent waves; wave w takes descriptors w, w + grid, ...
-//             (classes largest first, so every wave sees the same mix and the
-//             tail is made of the cheapest tiles).  A descriptor is located
-//             from the class table held in lanes 0..32 (ballot + readlane, no
-//             memory access), and the slot offsets of the next descriptor are
-//             loaded into lanes 0..15 while the current one is worked on, so
-//             no metadata load sits between a tile and its data loads.  The
-//             tile routines are the plain kernel's, with per-matrix gather
-//             loads and scatter stores.
+//   invert    one workgroup (one wave) per descriptor, classes largest first
+//             so the tail of the launch is made of the cheapest work.  A
+//             descriptor is located from the class table held in lanes 0..32
+//             (ballot + readlane; the 1.5 KB table is read by every wave and
+//             stays cached), its up to 64 slot offsets arrive with one vector
+//             load (one per lane), and its tiles run the plain kernel's tile
+//             routines with per-matrix gather loads and scatter stores; the
+//             loads of tile b+1 are issued before the stores of tile b.
 //
 // A matrix's result does not depend on which tile or slot it lands in: every
 // tile routine treats its matrices independently (the serial fallback of the

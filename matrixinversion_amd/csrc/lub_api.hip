@@ -50,8 +50,11 @@ std::mutex g_tuning_mutex;
 lub::Tuning g_tuning = [] {
   lub::Tuning t;
   for (int n = 0; n <= lub::kMaxN; ++n) {
+    // measured on MI355X (python -m matrixinversion_amd.autotune): quads win
+    // up to n = 12; ragged work items of 8 row tiles (4 at n = 32) beat
+    // single tiles by 5-10 % for n >= 13
     t.layout[n] = n <= 12 ? 1 : 0;
-    t.grab[n] = n <= 12 ? 1 : 2;
+    t.grab[n] = n <= 12 ? 1 : (n == 32 ? 4 : 8);
   }
   return t;
 }();
