@@ -269,6 +269,56 @@ def bench_ragged(args) -> int:
     return 0 if same and graph_same and bad == 0 else 3
 
 
+def serve_sweep(rates, duration: float, budget_gb: float = 64.0, pool: int = 200_000) -> list:
+    """Poisson serving runs (matrixinversion_amd.serve) at each rate."""
+    from matrixinversion_amd import serve
+
+    pools = (serve.Pool(42, pool), serve.Pool(43, pool, dominant=True))
+    out = []
+    with serve.Server(int(budget_gb * 2**30)) as srv:
+        for r in rates:
+            res = serve.poisson_stream(srv, pools, float(r), duration)
+            log(f"[serve] {r:>8.0f} req/s: p50 {res['p50_ms']:.3f} ms, p99 {res['p99_ms']:.3f} ms, "
+                f"{res['matrices_per_s'] / 1e6:.2f} M matrices/s, {res['matrices_per_batch']:.0f} matrices/batch")
+            out.append(res)
+    for p in pools:
+        p.close()
+    return out
+
+
+def bench_serve(args) -> int:
+    """Mixed serving stream: Poisson arrivals, 1..4096 matrices per request
+    (log-uniform), n ~ U{4..32}, 70 % pivoted U[-1,1) / 30 % unpivoted
+    diagonally dominant, 64 GB HBM admission budget.  One JSON line; value =
+    p99 latency at the reference rate (the last of --rates)."""
+    rates = [float(x) for x in args.rates.split(",")]
+    sweep = serve_sweep(rates, args.duration)
+    top = sweep[-1]
+    out = {
+        "metric": "p50/p99 ms, mixed serving stream",
+        "value": top["p99_ms"],
+        "unit": "ms (p99)",
+        "n_gpus": 1,
+        "steps": len(sweep),
+        "warmup": 0,
+        "ms_per_step": top["mean_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: request pools from mt19937(42/43), Poisson arrivals from RandomState(7)",
+        "config": {"workload": "serving: Poisson arrivals, log-uniform 1..4096 matrices per request, n~U{4..32}, "
+                               "70% pivoted / 30% unpivoted dominant, 64 GB admission budget",
+                   "rates_req_s": rates, "duration_s": args.duration},
+        "detail": {"p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "matrices_per_s": top["matrices_per_s"],
+                   "sweep": sweep},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,8 +330,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n-lo", type=int, default=2, help="ragged: smallest n")
     ap.add_argument("--n-hi", type=int, default=32, help="ragged: largest n")
-    ap.add_argument("--workload", default="headline", choices=["headline", "ragged"],
-                    help="headline: the metric's configuration; ragged: mixed-n batch (--batch defaults to 500k)")
+    ap.add_argument("--workload", default="headline", choices=["headline", "ragged", "serve"],
+                    help="headline: the metric's configuration; ragged: mixed-n batch (--batch defaults to 500k); "
+                         "serve: Poisson serving sweep")
+    ap.add_argument("--rates", default="1000,4000,16000", help="serve: request rates (req/s)")
+    ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per rate")
+    ap.add_argument("--no-serve", action="store_true", help="headline: skip the short serving run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -291,6 +345,8 @@ def main() -> int:
 
     if args.workload == "ragged":
         return bench_ragged(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import matrixinversion_amd as mia
     from matrixinversion_amd import workloads
@@ -377,16 +433,23 @@ def main() -> int:
                    "algorithmic_bytes_per_launch": algo_bytes, "pct_hbm": 100 * achieved / HBM_PEAK_GBPS,
                    "verifyInv_correct": correct, "max_abs_AXmI": worst},
     }
+    for b in bufs:
+        b.close()
+    pristine.close()
+    piv.close()
+    if not args.no_serve:
+        # the metric's second half: latency of the mixed serving stream
+        top = serve_sweep([16000.0], 2.0)[0]
+        out["serving"] = {"rate_req_s": top["rate_req_s"], "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
+                          "matrices_per_s": top["matrices_per_s"],
+                          "workload": "Poisson arrivals, log-uniform 1..4096 matrices/request, n~U{4..32}, "
+                                      "70% pivoted / 30% unpivoted dominant, 64 GB admission budget"}
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(n)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
-    for b in bufs:
-        b.close()
-    pristine.close()
-    piv.close()
     return 0
 
 

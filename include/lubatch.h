@@ -39,6 +39,8 @@ extern "C" {
 #define LUB_ENODEVICE     6   /* no gfx950 device / library not initialised   */
 #define LUB_EALIGN        7   /* pointer not 4-byte aligned                   */
 #define LUB_EINVAL_ARG    8   /* other invalid argument (ragged tables, ...)  */
+#define LUB_EBUSY         9   /* server: no room under the budget right now;
+                                 retry after lu_server_progress / _release    */
 
 #define LUB_MAX_N 32
 
@@ -123,6 +125,45 @@ int lu_ragged_invalid_count(const lub_ragged_plan* plan, int* out);
  * (entry 0 unused). */
 int lu_tuning_set(const signed char* layout, const signed char* grab);
 int lu_tuning_get(signed char* layout, signed char* grab);
+
+/* ---- serving ------------------------------------------------------------
+ * A continuous-batching server over the ragged path.  The reference has no
+ * serving layer (one batch per process: parallel_pivot/run.py:198-234).
+ *
+ * Requests are admitted into a device arena of budget_bytes (the HBM
+ * admission budget) and stay resident until released.  Queued requests of
+ * one pivot rule are coalesced into one ragged batch whenever that rule's
+ * stream is idle; requests of different rules run on their own streams.
+ * A result is bit-identical to inverting the same matrices alone.
+ * One host thread drives a server; no call blocks except destroy. */
+typedef struct lub_server lub_server;
+
+/* max_batch: most matrices in one launched batch (and in one request);
+ * ring_slots (>= max_batch): queued + running matrices per pivot rule. */
+int lu_server_create(lub_server** server, int64_t budget_bytes, int64_t max_batch, int64_t ring_slots);
+int lu_server_destroy(lub_server* server);
+
+/* Admit `count` matrices of sizes ns[0..count) (host array), packed
+ * row-major back to back at src (device or host memory), to be inverted
+ * with `flags` (pivot rule | layout).  The data is copied in asynchronously;
+ * src may be reused once the request is done.  LUB_EBUSY: not now. */
+int lu_server_submit(lub_server* server, const float* src, const int* ns, int count, int flags, uint64_t* ticket);
+
+/* Retire finished batches, launch queued work on idle streams; counts of
+ * launched batches and completed requests (either pointer may be NULL). */
+int lu_server_progress(lub_server* server, int* launched, int* completed);
+
+/* state: 0 queued, 1 running, 2 done.  result / piv: device pointers to the
+ * request's inverses (same packing as the input) and pivot vectors (n ints
+ * per matrix; NULL for LUB_PIVOT_NONE); valid until release.  latency_ms:
+ * submit to the progress call that saw completion (-1 before done). */
+int lu_server_query(lub_server* server, uint64_t ticket, int* state, float** result, int** piv, double* latency_ms);
+
+/* Give a done request's memory back. */
+int lu_server_release(lub_server* server, uint64_t ticket);
+
+/* Matrices queued and running; arena bytes held (admitted, not reclaimed). */
+int lu_server_stats(lub_server* server, int64_t* queued, int64_t* running, int64_t* resident_bytes);
 
 /* Device-side inversion check, the reference's verifyInv (parallel_pivot/
  * verify.hpp:50-103) on the GPU.  For each of `batch` matrices, A (the

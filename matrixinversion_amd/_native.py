@@ -28,6 +28,14 @@ ABI = {
     "lu_batched_inverse_ragged": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "lu_ragged_invalid_count": (_i, [_vp, ctypes.POINTER(_i)]),
     "lu_tuning_set": (_i, [ctypes.c_char_p, ctypes.c_char_p]),
+    "lu_server_create": (_i, [ctypes.POINTER(_vp), _i64, _i64, _i64]),
+    "lu_server_destroy": (_i, [_vp]),
+    "lu_server_submit": (_i, [_vp, _vp, _vp, _i, _i, ctypes.POINTER(ctypes.c_uint64)]),
+    "lu_server_progress": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "lu_server_query": (_i, [_vp, ctypes.c_uint64, ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                             ctypes.POINTER(ctypes.c_double)]),
+    "lu_server_release": (_i, [_vp, ctypes.c_uint64]),
+    "lu_server_stats": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "lu_tuning_get": (_i, [ctypes.c_char_p, ctypes.c_char_p]),
     "lu_status_string": (ctypes.c_char_p, [_i]),
     "lu_last_error": (ctypes.c_char_p, []),

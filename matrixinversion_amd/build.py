@@ -52,11 +52,23 @@ def _run(cmd):
     return time.time() - t0, err
 
 
-def build(jobs: int = 8, verbose: bool = False) -> str:
+def _stale(cmd, deps_mtime: float) -> bool:
+    obj = cmd[-1]
+    return not os.path.exists(obj) or os.path.getmtime(obj) < deps_mtime
+
+
+def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every unit and link; with `incremental`, units whose object is
+    newer than every source and header are kept (a developer shortcut: the
+    default always rebuilds)."""
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
-    subprocess.run([sys.executable, os.path.join(CSRC, "gen_gj_asm.py"), os.path.join(CSRC, "gj_asm.inc")],
-                   check=True)
+    inc = os.path.join(CSRC, "gj_asm.inc")
+    subprocess.run([sys.executable, os.path.join(CSRC, "gen_gj_asm.py"), inc + ".new"], check=True)
+    if os.path.exists(inc) and open(inc).read() == open(inc + ".new").read():
+        os.remove(inc + ".new")  # unchanged: keep its timestamp
+    else:
+        os.replace(inc + ".new", inc)
     units = []
     for lo, hi in SIZE_RANGES:
         obj = os.path.join(OBJDIR, f"lu_kernels_{lo}_{hi}.o")
@@ -65,13 +77,21 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     for piv in range(4):
         units.append([HIPCC, *CFLAGS, f"-DLUB_RAGGED_PIV={piv}", "-c", os.path.join(CSRC, "lu_ragged_inst.hip"),
                       "-o", os.path.join(OBJDIR, f"lu_ragged_{piv}.o")])
-    for name in ("lub_api", "lub_util", "lub_verify", "lub_ragged"):
+    for name in ("lub_api", "lub_util", "lub_verify", "lub_ragged", "lub_serve"):
         units.append([HIPCC, *CFLAGS, "-c", os.path.join(CSRC, f"{name}.hip"),
                       "-o", os.path.join(OBJDIR, f"{name}.o")])
+    todo = units
+    if incremental:
+        deps = [os.path.join(d, f) for d in (CSRC, INCLUDE) for f in os.listdir(d)
+                if f.endswith((".hip", ".hpp", ".h", ".inc"))]
+        newest_header = max(os.path.getmtime(f) for f in deps if not f.endswith(".hip"))
+        todo = [u for u in units
+                if _stale(u, max(newest_header, os.path.getmtime(next(a for a in u if a.endswith(".hip")))))]
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        results = list(ex.map(_run, units))
+        results = list(ex.map(_run, todo))
+    units_run = todo
     if verbose:
-        for cmd, (dt, err) in zip(units, results):
+        for cmd, (dt, err) in zip(units_run, results):
             print(f"[{dt:6.1f}s] {os.path.basename(cmd[-1])}" + (f"\n{err}" if err.strip() else ""))
     objs = [u[-1] for u in units]
     tmp = LIB + ".tmp"
@@ -84,9 +104,10 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 1))
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true", help="keep objects newer than their sources")
     args = ap.parse_args()
     t0 = time.time()
-    lib = build(args.jobs, args.verbose)
+    lib = build(args.jobs, args.verbose, args.incremental)
     print(f"built {lib} in {time.time() - t0:.1f}s")
 
 

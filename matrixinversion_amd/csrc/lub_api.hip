@@ -230,6 +230,7 @@ const char* lu_status_string(int status) {
     case LUB_ENODEVICE: return "no kernel for this device";
     case LUB_EALIGN: return "pointer not 4-byte aligned";
     case LUB_EINVAL_ARG: return "invalid argument";
+    case LUB_EBUSY: return "no room under the server budget now";
     default: return "unknown status";
   }
 }

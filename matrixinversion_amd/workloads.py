@@ -58,11 +58,11 @@ def ragged_sizes(seed: int, batch: int, n_lo: int = 2, n_hi: int = 32) -> np.nda
     return (n_lo + raw % np.uint32(n_hi - n_lo + 1)).astype(np.int32)
 
 
-def ragged_batch(seed: int, batch: int, n_lo: int = 2, n_hi: int = 32):
+def ragged_batch(seed: int, batch: int, n_lo: int = 2, n_hi: int = 32, dominant: bool = False):
     """A packed ragged batch: sizes from mt19937(seed + 1), entries U[-1,1)
-    from mt19937(seed) in matrix order.  Returns (flat A, offsets int64,
-    ns int32, pivot offsets int64); matrix i occupies
-    A[offsets[i] : offsets[i] + ns[i]**2] (row-major)."""
+    from mt19937(seed) in matrix order (+ n*I per matrix when `dominant`).
+    Returns (flat A, offsets int64, ns int32, pivot offsets int64); matrix i
+    occupies A[offsets[i] : offsets[i] + ns[i]**2] (row-major)."""
     ns = ragged_sizes(seed + 1, batch, n_lo, n_hi)
     sq = ns.astype(np.int64) ** 2
     offsets = np.zeros(batch, np.int64)
@@ -70,6 +70,11 @@ def ragged_batch(seed: int, batch: int, n_lo: int = 2, n_hi: int = 32):
     poffs = np.zeros(batch, np.int64)
     np.cumsum(ns[:-1].astype(np.int64), out=poffs[1:])
     flat = mt19937_uniform(seed, int(sq.sum()))
+    if dominant and batch:
+        n64 = ns.astype(np.int64)
+        k = np.arange(int(n64.sum())) - np.repeat(poffs, n64)       # 0..n_i-1 within matrix i
+        diag = np.repeat(offsets, n64) + k * (np.repeat(n64, n64) + 1)
+        flat[diag] += np.repeat(ns, n64).astype(np.float32)
     return flat, offsets, ns, poffs
 
 

@@ -102,6 +102,24 @@ def test_ragged_and_tuning_validation_codes():
     assert (api.tuning_get()[1] == grab).all()
 
 
+def test_server_validation_codes():
+    import ctypes
+
+    from matrixinversion_amd import _native
+
+    lib = _native.lib()
+    h = ctypes.c_void_p()
+    t = ctypes.c_uint64()
+    assert lib.lu_server_create(None, 1 << 20, 16, 16) == 3
+    assert lib.lu_server_create(ctypes.byref(h), 100, 16, 16) == 8        # budget below one block
+    assert lib.lu_server_create(ctypes.byref(h), 1 << 20, 16, 8) == 8     # ring smaller than a batch
+    assert lib.lu_server_destroy(None) == 3
+    assert lib.lu_server_submit(None, 0x1000, 0x1000, 1, 1, ctypes.byref(t)) == 3
+    assert lib.lu_server_progress(None, None, None) == 3
+    assert lib.lu_server_release(None, 1) == 3
+    assert lib.lu_status_string(9).decode().startswith("no room")
+
+
 def test_pack_ragged_host_layout():
     import numpy as np
 

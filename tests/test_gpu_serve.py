@@ -1,0 +1,125 @@
+"""GPU tests of the serving engine (lu_server_*): every request's result is
+bit-identical to inverting its matrices alone (plain launches per n), under
+any interleaving of pivot rules and sizes, with and without admission
+back-pressure.  Every case prints its name before touching the GPU."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(mia, host_flat, ns, flags):
+    """Inverse and pivots of each matrix through the plain entry point, one
+    batch per n (the 'alone' reference)."""
+    offs = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)[:-1]])
+    x = np.empty_like(host_flat)
+    p = [None] * ns.size
+    for n in np.unique(ns):
+        sel = np.nonzero(ns == n)[0]
+        a = np.stack([host_flat[offs[i]:offs[i] + n * n].reshape(n, n) for i in sel])
+        xi, pi = mia.invert(a, flags)
+        for k, i in enumerate(sel):
+            x[offs[i]:offs[i] + n * n] = xi[k].ravel()
+            p[i] = pi[k]
+    return x, p
+
+
+def _check_request(mia, serve, srv, t, pool, start, count, flags):
+    st, res, piv, lat = srv.query(t)
+    assert st == serve.DONE and lat >= 0
+    ns = pool.ns[start:start + count]
+    host = pool.host(start, count)
+    x = serve.download(res, host.nbytes, np.float32)
+    xa, pa = _alone(mia, host, ns, flags)
+    assert np.array_equal(x.view(np.uint32), xa.view(np.uint32)), "result differs from inverting alone"
+    if flags != 0:
+        pv = serve.download(piv, int(ns.sum()) * 4, np.int32)
+        assert np.array_equal(pv, np.concatenate(pa)), "pivots differ from inverting alone"
+        offs = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)[:-1]])
+        for k in range(min(count, 8)):
+            n = int(ns[k])
+            a = host[offs[k]:offs[k] + n * n].reshape(1, n, n)
+            assert (oracle.pivots(a[0], oracle.PIV_FIRSTMAX) == pa[k]).all()
+
+
+def test_mixed_requests_match_alone(gpu):
+    from matrixinversion_amd import serve
+
+    pools = (serve.Pool(42, 6000), serve.Pool(43, 6000, dominant=True))
+    counts, starts, pivot = serve.request_mix(3, 60, 6000, max_count=300)
+    reqs = {}
+    print("[case] serve mixed 60 requests", flush=True)
+    with serve.Server(256 << 20, max_batch=1 << 16, ring_slots=1 << 17) as srv:
+        for i in range(60):
+            pool = pools[0] if pivot[i] else pools[1]
+            flags = 0x1 if pivot[i] else 0x0
+            ptr, ns = pool.request(int(starts[i]), int(counts[i]))
+            t = srv.submit(ptr, ns, flags)
+            assert t is not None
+            reqs[t] = (pool, int(starts[i]), int(counts[i]), flags)
+            if i % 7 == 0:
+                srv.progress()
+        for _ in range(100000):
+            srv.progress()
+            if all(srv.query(t)[0] == serve.DONE for t in reqs):
+                break
+        for t, (pool, s0, c, f) in reqs.items():
+            _check_request(gpu, serve, srv, t, pool, s0, c, f)
+        for t in reqs:
+            srv.release(t)
+        q, r, b = srv.stats()
+        assert q == 0 and r == 0 and b == 0
+    for p in pools:
+        p.close()
+
+
+def test_budget_backpressure(gpu):
+    from matrixinversion_amd import serve
+
+    pool = serve.Pool(44, 3000)
+    counts, starts, _ = serve.request_mix(5, 40, 3000, max_count=200)
+    print("[case] serve budget 1 MiB", flush=True)
+    busy = 0
+    with serve.Server(1 << 20, max_batch=4096, ring_slots=8192) as srv:
+        pending = list(range(40))
+        live = {}
+        checked = 0
+        for _ in range(1000000):
+            while pending:
+                i = pending[0]
+                ptr, ns = pool.request(int(starts[i]), int(counts[i]))
+                t = srv.submit(ptr, ns, 0x1)
+                if t is None:
+                    busy += 1
+                    break
+                live[t] = i
+                pending.pop(0)
+            srv.progress()
+            for t in list(live):
+                if srv.query(t)[0] == serve.DONE:
+                    i = live.pop(t)
+                    if checked < 10:
+                        _check_request(gpu, serve, srv, t, pool, int(starts[i]), int(counts[i]), 0x1)
+                        checked += 1
+                    srv.release(t)
+            if not pending and not live:
+                break
+        assert not pending and not live
+        assert srv.stats()[2] == 0
+    assert busy > 0, "a 1 MiB budget should have pushed back at least once"
+    pool.close()
+
+
+def test_poisson_stream_short(gpu):
+    from matrixinversion_amd import serve
+
+    pools = (serve.Pool(42, 20000), serve.Pool(43, 20000, dominant=True))
+    print("[case] serve poisson 2000 req/s for 0.3 s", flush=True)
+    with serve.Server(4 << 30) as srv:
+        res = serve.poisson_stream(srv, pools, 2000.0, 0.3)
+    print(res, flush=True)
+    assert res["requests"] > 0 and np.isfinite(res["p99_ms"]) and res["p50_ms"] > 0
+    for p in pools:
+        p.close()

@@ -54,3 +54,14 @@ def test_ragged_batch_layout():
     assert (poffs[1:] == poffs[:-1] + ns[:-1]).all()
     assert (flat == workloads.mt19937_uniform(42, flat.size)).all()
     assert workloads.ragged_bytes(ns) == int((8 * sq + 4 * ns + 20).sum())
+
+
+def test_ragged_batch_dominant():
+    from matrixinversion_amd import workloads
+
+    flat, offs, ns, _ = workloads.ragged_batch(3, 50, 4, 9)
+    dflat, _, _, _ = workloads.ragged_batch(3, 50, 4, 9, dominant=True)
+    for o, n in zip(offs, ns):
+        a = flat[o:o + n * n].reshape(n, n)
+        d = dflat[o:o + n * n].reshape(n, n)
+        assert (d == a + np.float32(n) * np.eye(n, dtype=np.float32)).all()
