@@ -319,6 +319,72 @@ def bench_serve(args) -> int:
     return 0
 
 
+def bench_refine(args) -> int:
+    """Precision workload: 100,000 matrices of n=32, A = Q1 diag(sigma) Q2^T,
+    sigma log-spaced 1..10^k for k in {2, 4, 6}; plain FP32 LU inverse vs one
+    refinement step (FP64 residual, FP32 MFMA correction), relative error
+    against the FP64 inverse of the same FP32 A.  One JSON line; value = the
+    median relative error after refinement at k = 4."""
+    import matrixinversion_amd as mia
+    from matrixinversion_amd import workloads
+
+    batch, n = args.batch or 100_000, 32
+    stream = mia.Stream()
+    per_k = {}
+    for k in (2, 4, 6):
+        t0 = time.perf_counter()
+        a = workloads.conditioned_matrices(100 + k, batch, n, k)
+        log(f"[refine] k={k}: generated {batch} matrices in {time.perf_counter() - t0:.1f}s")
+        A = mia.DeviceBuffer.from_array(a)
+        X = mia.DeviceBuffer(a.nbytes)
+        P = mia.DeviceBuffer(batch * n * 4)
+        e = [mia.Event() for _ in range(3)]
+        t_inv, t_ref = [], []
+        for _ in range(max(1, args.steps)):
+            X.copy_from(A, stream)
+            e[0].record(stream)
+            mia.lu_batched_inverse(X, n, batch, mia.PIVOT_PARALLEL, P, stream)
+            e[1].record(stream)
+            if _ == 0:
+                stream.synchronize()
+                x0 = X.download(a.shape, np.float32)
+                e[1].record(stream)
+            mia.refine(A, X, n, batch, stream)
+            e[2].record(stream)
+            stream.synchronize()
+            t_inv.append(e[0].elapsed_ms(e[1]) if _ else float("nan"))
+            t_ref.append(e[1].elapsed_ms(e[2]))
+        x1 = X.download(a.shape, np.float32)
+        for b in (A, X, P):
+            b.close()
+        inv = np.linalg.inv(a.astype(np.float64))
+        nrm = np.linalg.norm(inv, axis=(1, 2))
+        e0 = np.linalg.norm(x0 - inv, axis=(1, 2)) / nrm
+        e1 = np.linalg.norm(x1 - inv, axis=(1, 2)) / nrm
+        ref_ms = float(np.mean(t_ref))
+        per_k[k] = {"median_rel_err_plain": float(np.median(e0)), "median_rel_err_refined": float(np.median(e1)),
+                    "p99_rel_err_plain": float(np.percentile(e0, 99)),
+                    "p99_rel_err_refined": float(np.percentile(e1, 99)),
+                    "refine_ms": ref_ms, "inverse_ms": float(np.nanmean(t_inv)) if len(t_inv) > 1 else None,
+                    "refine_GBps": batch * 12 * n * n / (ref_ms * 1e-3) / 1e9}
+        log(f"[refine] k={k}: median rel err {np.median(e0):.2e} -> {np.median(e1):.2e}, "
+            f"p99 {np.percentile(e0, 99):.2e} -> {np.percentile(e1, 99):.2e}; refine {ref_ms:.3f} ms")
+    out = {
+        "metric": "relative error vs FP64 inverse after one refinement step (median, k=4)",
+        "value": per_k[4]["median_rel_err_refined"],
+        "unit": "relative Frobenius error",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": per_k[4]["refine_ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 residual)",
+        "data": "synthetic: A = Q1 diag(logspace(0,k,32)) Q2^T, Q from QR of RandomState(100+k) Gaussians",
+        "config": {"workload": f"refinement, {batch} x n=32, k in (2, 4, 6), pivot=parallel", "batch": batch},
+        "detail": per_k,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,9 +396,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n-lo", type=int, default=2, help="ragged: smallest n")
     ap.add_argument("--n-hi", type=int, default=32, help="ragged: largest n")
-    ap.add_argument("--workload", default="headline", choices=["headline", "ragged", "serve"],
+    ap.add_argument("--workload", default="headline", choices=["headline", "ragged", "serve", "refine"],
                     help="headline: the metric's configuration; ragged: mixed-n batch (--batch defaults to 500k); "
-                         "serve: Poisson serving sweep")
+                         "serve: Poisson serving sweep; refine: precision workload (100k n=32)")
     ap.add_argument("--rates", default="1000,4000,16000", help="serve: request rates (req/s)")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per rate")
     ap.add_argument("--no-serve", action="store_true", help="headline: skip the short serving run")
@@ -347,6 +413,8 @@ def main() -> int:
         return bench_ragged(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "refine":
+        return bench_refine(args)
 
     import matrixinversion_amd as mia
     from matrixinversion_amd import workloads

@@ -126,6 +126,19 @@ int lu_ragged_invalid_count(const lub_ragged_plan* plan, int* out);
 int lu_tuning_set(const signed char* layout, const signed char* grab);
 int lu_tuning_get(signed char* layout, signed char* grab);
 
+/* ---- refinement ---------------------------------------------------------
+ * One step of iterative refinement, X <- X + X (I - A X), for `batch` n x n
+ * FP32 matrices (A and X contiguous, row-major): the residual I - A X is
+ * accumulated in FP64, the correction X (I - A X) comes from FP32 matrix-core
+ * (MFMA) products.  The reference has no refinement step: its inverse is the
+ * FP32 LU result (parallel_pivot/luBatchedInplace.cuh:84-125). */
+int lu_batched_refine(const float* A, float* X, int n, int64_t batch, void* stream);
+
+/* Out-of-place inverse plus one refinement step: X <- A^-1 (the plain path
+ * with `flags`, pivots to piv as in lu_batched_inverse), then
+ * lu_batched_refine(A, X).  A is left unchanged (A != X). */
+int lu_batched_inverse_refined(const float* A, float* X, int* piv, int n, int64_t batch, int flags, void* stream);
+
 /* ---- serving ------------------------------------------------------------
  * A continuous-batching server over the ragged path.  The reference has no
  * serving layer (one batch per process: parallel_pivot/run.py:198-234).

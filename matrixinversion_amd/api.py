@@ -98,6 +98,39 @@ def invert(a: np.ndarray, flags: int = PIVOT_PARALLEL, return_pivots: bool = Tru
             P.close()
 
 
+def refine(A: DeviceBuffer, X: DeviceBuffer, n: int, batch: int, stream: Stream | None = None) -> None:
+    """X <- X + X (I - A X): FP64 residual, FP32 MFMA correction
+    (lu_batched_refine).  Waits for completion when no stream is given."""
+    if batch * n * n * 4 > min(A.nbytes, X.nbytes):
+        raise ValueError("A or X is smaller than batch*n*n floats")
+    check(lib().lu_batched_refine(A.ptr, X.ptr, n, batch, stream.handle if stream else None), "lu_batched_refine")
+    if stream is None:
+        from .device import synchronize
+        synchronize()
+
+
+def invert_refined(a: np.ndarray, flags: int = PIVOT_PARALLEL):
+    """Host convenience: (plain FP32 inverse, inverse after one refinement
+    step) of a (batch, n, n) FP32 array."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 2:
+        a = a[None]
+    batch, n, _ = a.shape
+    A = DeviceBuffer.from_array(a)
+    X = DeviceBuffer(a.nbytes)
+    P = DeviceBuffer(max(batch * n * 4, 4))
+    try:
+        X.copy_from(A)
+        lu_batched_inverse(X, n, batch, flags, P)
+        x0 = X.download(a.shape, np.float32)
+        refine(A, X, n, batch)
+        x1 = X.download(a.shape, np.float32)
+    finally:
+        for b in (A, X, P):
+            b.close()
+    return x0, x1
+
+
 class RaggedPlan:
     """Device workspace for ragged batches of up to `max_batch` matrices
     (lu_ragged_plan_create).  Enqueueing on a plan allocates nothing and never

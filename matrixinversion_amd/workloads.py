@@ -87,3 +87,15 @@ def ragged_bytes(ns: np.ndarray, pivots: bool = True) -> int:
     if pivots:
         per = per + 4 * n + 8
     return int(per.sum())
+
+
+def conditioned_matrices(seed: int, batch: int, n: int, k: float) -> np.ndarray:
+    """A = Q1 diag(sigma) Q2^T in FP64, rounded to FP32: Q1, Q2 from the QR
+    factorisation of seeded Gaussian matrices (RandomState(seed)), sigma
+    log-spaced from 1 to 10^k (2-norm condition number 10^k before rounding)."""
+    rng = np.random.RandomState(seed)
+    q1, _ = np.linalg.qr(rng.standard_normal((batch, n, n)))
+    q2, _ = np.linalg.qr(rng.standard_normal((batch, n, n)))
+    sigma = np.logspace(0.0, k, n)
+    a = np.einsum("bij,j,bkj->bik", q1, sigma, q2)
+    return a.astype(np.float32)

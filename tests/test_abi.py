@@ -102,6 +102,20 @@ def test_ragged_and_tuning_validation_codes():
     assert (api.tuning_get()[1] == grab).all()
 
 
+def test_refine_validation_codes():
+    from matrixinversion_amd import _native
+
+    lib = _native.lib()
+    fake = 0x1000
+    assert lib.lu_batched_refine(fake, fake, 0, 1, None) == 1
+    assert lib.lu_batched_refine(fake, fake, 33, 1, None) == 1
+    assert lib.lu_batched_refine(fake, fake, 4, -1, None) == 2
+    assert lib.lu_batched_refine(None, fake, 4, 1, None) == 3
+    assert lib.lu_batched_refine(fake + 2, fake, 4, 1, None) == 7
+    assert lib.lu_batched_refine(None, None, 4, 0, None) == 0
+    assert lib.lu_batched_inverse_refined(fake, fake, None, 4, 1, 1, None) == 8   # A == X
+
+
 def test_server_validation_codes():
     import ctypes
 
