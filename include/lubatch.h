@@ -126,6 +126,20 @@ int lu_ragged_invalid_count(const lub_ragged_plan* plan, int* out);
 int lu_tuning_set(const signed char* layout, const signed char* grab);
 int lu_tuning_get(signed char* layout, signed char* grab);
 
+/* ---- LAPACK-style outputs -----------------------------------------------
+ * The same inversion with getrf/getri-shaped outputs: ipiv[k] (1-based) is
+ * the row swapped with row k at step k, info is 0, or k + 1 when U(k,k) of
+ * P*A = LU is exactly zero (the inverse then holds inf/NaN).  The
+ * interchanges are those of the chosen pivot rule (the reference's first-max
+ * on the un-updated column: parallel_pivot/luBatchedInplace.cuh:12-44), so
+ * they may differ from LAPACK partial pivoting; applied in order they give
+ * this factorisation's P.  ipiv: batch x n ints, info: batch ints (device). */
+int lu_batched_inverse_lapack(float* A, int* ipiv, int* info, int n, int64_t batch, int flags, void* stream);
+
+/* In-place conversion of permutation pivot vectors (the `piv` output of
+ * lu_batched_inverse*) to 1-based interchange sequences. */
+int lu_perm_to_ipiv(int* piv, int n, int64_t batch, void* stream);
+
 /* ---- refinement ---------------------------------------------------------
  * One step of iterative refinement, X <- X + X (I - A X), for `batch` n x n
  * FP32 matrices (A and X contiguous, row-major): the residual I - A X is

@@ -29,6 +29,8 @@ ABI = {
     "lu_ragged_invalid_count": (_i, [_vp, ctypes.POINTER(_i)]),
     "lu_tuning_set": (_i, [ctypes.c_char_p, ctypes.c_char_p]),
     "lu_batched_refine": (_i, [_vp, _vp, _i, _i64, _vp]),
+    "lu_batched_inverse_lapack": (_i, [_vp, _vp, _vp, _i, _i64, _i, _vp]),
+    "lu_perm_to_ipiv": (_i, [_vp, _i, _i64, _vp]),
     "lu_batched_inverse_refined": (_i, [_vp, _vp, _vp, _i, _i64, _i, _vp]),
     "lu_server_create": (_i, [ctypes.POINTER(_vp), _i64, _i64, _i64]),
     "lu_server_destroy": (_i, [_vp]),

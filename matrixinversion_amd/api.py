@@ -98,6 +98,26 @@ def invert(a: np.ndarray, flags: int = PIVOT_PARALLEL, return_pivots: bool = Tru
             P.close()
 
 
+def invert_lapack(a: np.ndarray, flags: int = PIVOT_PARALLEL):
+    """Host convenience with getrf/getri-shaped outputs: (inverses, ipiv
+    (1-based interchanges, batch x n), info (batch))."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 2:
+        a = a[None]
+    batch, n, _ = a.shape
+    A = DeviceBuffer.from_array(a)
+    P = DeviceBuffer(max(batch * n * 4, 4))
+    I = DeviceBuffer(max(batch * 4, 4))
+    try:
+        check(lib().lu_batched_inverse_lapack(A.ptr, P.ptr, I.ptr, n, batch, flags, None), "lu_batched_inverse_lapack")
+        from .device import synchronize
+        synchronize()
+        return (A.download(a.shape, np.float32), P.download((batch, n), np.int32), I.download((batch,), np.int32))
+    finally:
+        for b in (A, P, I):
+            b.close()
+
+
 def refine(A: DeviceBuffer, X: DeviceBuffer, n: int, batch: int, stream: Stream | None = None) -> None:
     """X <- X + X (I - A X): FP64 residual, FP32 MFMA correction
     (lu_batched_refine).  Waits for completion when no stream is given."""

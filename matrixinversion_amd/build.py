@@ -77,7 +77,7 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
     for piv in range(4):
         units.append([HIPCC, *CFLAGS, f"-DLUB_RAGGED_PIV={piv}", "-c", os.path.join(CSRC, "lu_ragged_inst.hip"),
                       "-o", os.path.join(OBJDIR, f"lu_ragged_{piv}.o")])
-    for name in ("lub_api", "lub_util", "lub_verify", "lub_ragged", "lub_serve", "lub_refine"):
+    for name in ("lub_api", "lub_util", "lub_verify", "lub_ragged", "lub_serve", "lub_refine", "lub_lapack"):
         units.append([HIPCC, *CFLAGS, "-c", os.path.join(CSRC, f"{name}.hip"),
                       "-o", os.path.join(OBJDIR, f"{name}.o")])
     todo = units

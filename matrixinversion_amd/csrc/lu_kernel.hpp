@@ -425,15 +425,19 @@ __device__ __forceinline__ bool pivot_parallel(const ColBlock<N, L>& w, LaneInts
 //     and every later row operation commutes with that column scale; its
 //     pivot entry becomes R[k][k] / sigma_k = -1.
 // At the end (P*A)^-1 = S * diag(sigma); sigma of the lane's columns is cs[].
+// Returns the first step whose pivot is exactly zero (U(k,k) = 0 of P*A = LU,
+// LAPACK's info - 1), or N when there is none.
 template <int N, int L>
-__device__ __forceinline__ void gauss_jordan(ColBlock<N, L>& w, LaneFloats<N, L>& cs, int l) {
+__device__ __forceinline__ int gauss_jordan(ColBlock<N, L>& w, LaneFloats<N, L>& cs, int l) {
   using G = Geo<N, L>;
 #pragma unroll
   for (int h = 0; h < G::R; ++h) cs[h] = 1.0f;
+  int zero_at = N;
   static_for<0, N>([&](auto K) {
     constexpr int k = decltype(K)::value;
     constexpr int kl = k % L, kh = k / L;
     const float d = dpp_f32<bcast_ctrl<L, kl>()>(w[kh][k]);
+    zero_at = (d == 0.0f && zero_at == N) ? k : zero_at;
     float r = __builtin_amdgcn_rcpf(d);
     r = __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);  // one Newton step
     const bool pl = (l == kl);                              // this lane owns column k (set kh)
@@ -453,6 +457,7 @@ __device__ __forceinline__ void gauss_jordan(ColBlock<N, L>& w, LaneFloats<N, L>
     for (int h = 0; h < G::R; ++h) w[h][k] = g[h];
     w[kh][k] = pl ? -1.0f : g[kh];
   });
+  return zero_at;
 }
 
 // ------------------------------------------------------------ the kernel
@@ -511,8 +516,9 @@ __device__ __forceinline__ void tile_front(ColBlock<N, L>& w, LaneInts<N, L>& pr
 
 // Phases 6-7: (P*A)^-1 in registers, column q written to column perm[q] of
 // the matrix's LDS region, which then holds A^-1 (row-major, stride LD).
+// Returns gauss_jordan's first zero-pivot step (N: none) of this lane's matrix.
 template <int N, int L>
-__device__ __forceinline__ void tile_solve(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* lds, int lane) {
+__device__ __forceinline__ int tile_solve(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* lds, int lane) {
   using G = Geo<N, L>;
   const int l = lane % L;
   const int g = lane / L;
@@ -520,7 +526,7 @@ __device__ __forceinline__ void tile_solve(ColBlock<N, L>& w, const LaneInts<N, 
 
   // 6. (P*A)^-1 = S * diag(cs), column q in lane q
   LaneFloats<N, L> cs;
-  gauss_jordan<N, L>(w, cs, l);
+  const int zero_at = gauss_jordan<N, L>(w, cs, l);
 
   // 7. column q of (P*A)^-1 is column perm[q] of A^-1
 #pragma unroll
@@ -532,18 +538,20 @@ __device__ __forceinline__ void tile_solve(ColBlock<N, L>& w, const LaneInts<N, 
     }
   }
   wave_sync();
+  return zero_at;
 }
 
 // Phase 8 for a contiguous tile: the inverses and the pivot vectors to HBM.
 template <int N, int L>
 __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* __restrict__ A,
-                                          int* __restrict__ piv, int64_t m0, int valid, bool vec, float* lds,
-                                          int lane) {
+                                          int* __restrict__ piv, int* __restrict__ info, int64_t m0, int valid,
+                                          bool vec, float* lds, int lane) {
   using G = Geo<N, L>;
   const int l = lane % L;
   const int g = lane / L;
-  tile_solve<N, L>(w, prm, lds, lane);
+  const int zero_at = tile_solve<N, L>(w, prm, lds, lane);
   tile_store<N, L>(A + m0 * (N * N), lds, valid, vec, lane);
+  if (info != nullptr && l == 0 && g < valid) info[m0 + g] = zero_at < N ? zero_at + 1 : 0;
   if (piv != nullptr && g < valid) {
 #pragma unroll
     for (int h = 0; h < G::R; ++h)
@@ -560,7 +568,7 @@ __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L
 // (the LDS tile of n = 32 allows no more than that anyway).
 template <int N, int L, int PIV>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void lu_inverse_kernel(
-    float* __restrict__ A, int* __restrict__ piv, int64_t batch, int64_t ntiles, int vec) {
+    float* __restrict__ A, int* __restrict__ piv, int* __restrict__ info, int64_t batch, int64_t ntiles, int vec) {
   using G = Geo<N, L>;
   __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
   const int lane = threadIdx.x;
@@ -583,7 +591,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
     const int64_t tn = t + stride;
     have = tn < ntiles && full(tn);
     if (have) tile_fetch<N, L>(A + tn * G::MPW * (N * N), next, lane);
-    tile_back<N, L>(w, prm, A, piv, m0, valid, vec != 0, lds, lane);
+    tile_back<N, L>(w, prm, A, piv, info, m0, valid, vec != 0, lds, lane);
   }
 }
 

@@ -152,7 +152,35 @@ Tuning current_tuning() {
   return g_tuning;
 }
 const KernelTable& kernel_table() { return table(); }
+
+// The plain launch; info (nullable) receives LAPACK-style per-matrix info.
+int enqueue_plain(float* A, int* piv, int* info, int n, int64_t batch, int flags, void* stream) {
+  int rc = validate(A, piv, n, batch, flags);
+  if (rc != LUB_OK || batch == 0) return rc;
+  const int layout = choose_layout(n, flags);
+  const void* fn = table().plain[n][flags & LUB_PIVOT_MASK][layout];
+  if (fn == nullptr) return LUB_ENODEVICE;
+  const int per_tile = 64 / lub::kLayoutLanes[layout];  // matrices per wave tile
+  int64_t ntiles = (batch + per_tile - 1) / per_tile;
+  int64_t cap = kMaxGrid;
+  if (int t = env_tiles_per_wg(); t > 1) cap = std::min<int64_t>(cap, (ntiles + t - 1) / t);
+  if (int wpc = env_waves_per_cu(); wpc > 0) {
+    cap = resident_workgroups(fn, n, flags & LUB_PIVOT_MASK, layout, wpc);
+    if (cap <= 0) return LUB_ENODEVICE;
+  }
+  unsigned grid = static_cast<unsigned>(std::min<int64_t>(ntiles, cap));
+  int vec = (reinterpret_cast<uintptr_t>(A) & 15u) == 0 ? 1 : 0;
+  void* args[] = {&A, &piv, &info, &batch, &ntiles, &vec};
+  hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) {
+    set_last_error("hipLaunchKernel", e);
+    return LUB_EHIP;
+  }
+  return LUB_OK;
+}
 }  // namespace lub
+
+
 
 extern "C" {
 
@@ -180,28 +208,7 @@ int lu_tuning_get(signed char* layout, signed char* grab) {
 }
 
 int lu_batched_inverse_async64(float* A, int* piv, int n, int64_t batch, int flags, void* stream) {
-  int rc = validate(A, piv, n, batch, flags);
-  if (rc != LUB_OK || batch == 0) return rc;
-  const int layout = choose_layout(n, flags);
-  const void* fn = table().plain[n][flags & LUB_PIVOT_MASK][layout];
-  if (fn == nullptr) return LUB_ENODEVICE;
-  const int per_tile = 64 / lub::kLayoutLanes[layout];  // matrices per wave tile
-  int64_t ntiles = (batch + per_tile - 1) / per_tile;
-  int64_t cap = kMaxGrid;
-  if (int t = env_tiles_per_wg(); t > 1) cap = std::min<int64_t>(cap, (ntiles + t - 1) / t);
-  if (int wpc = env_waves_per_cu(); wpc > 0) {
-    cap = resident_workgroups(fn, n, flags & LUB_PIVOT_MASK, layout, wpc);
-    if (cap <= 0) return LUB_ENODEVICE;
-  }
-  unsigned grid = static_cast<unsigned>(std::min<int64_t>(ntiles, cap));
-  int vec = (reinterpret_cast<uintptr_t>(A) & 15u) == 0 ? 1 : 0;
-  void* args[] = {&A, &piv, &batch, &ntiles, &vec};
-  hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, static_cast<hipStream_t>(stream));
-  if (e != hipSuccess) {
-    set_last_error("hipLaunchKernel", e);
-    return LUB_EHIP;
-  }
-  return LUB_OK;
+  return lub::enqueue_plain(A, piv, nullptr, n, batch, flags, stream);
 }
 
 int lu_batched_inverse_async(float* A, int* piv, int n, int batch, int flags, void* stream) {

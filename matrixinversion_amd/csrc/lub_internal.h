@@ -24,10 +24,14 @@ struct KernelTable {
 // autotuner through lu_tuning_set, defaults in lub_api.hip).
 struct Tuning {
   int8_t layout[kMaxN + 1];  // 0: 16-lane rows, 1: quads
-  int8_t grab[kMaxN + 1];    // ragged: wave tiles per queue descriptor
+  int8_t grab[kMaxN + 1];    // ragged: wave tiles per work item
 };
 Tuning current_tuning();
 const KernelTable& kernel_table();
+
+// The plain launch behind lu_batched_inverse_async64; info (nullable, one int
+// per matrix) receives 0 or 1 + the first step with an exactly zero pivot.
+int enqueue_plain(float* A, int* piv, int* info, int n, int64_t batch, int flags, void* stream);
 
 }  // namespace lub
 

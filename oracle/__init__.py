@@ -248,3 +248,31 @@ def ref_verify_lu(pa: np.ndarray, lu: np.ndarray) -> int:
 def ref_calc_cond_num(a: np.ndarray) -> float:
     a, ap = _f32(a)
     return float(_load_ref().ref_calc_cond_num(ap, a.shape[-1]))
+
+
+def ipiv_from_perm(perm: np.ndarray) -> np.ndarray:
+    """LAPACK getrf-style pivots (1-based row interchanges) describing the
+    permutation `perm` (row i of P*A is row perm[i] of A): at step k the row
+    now holding perm[k] is swapped into position k.  Restates no reference
+    code (the reference never exports its `pivots`,
+    parallel_pivot/luBatchedInplace.cuh:145-148); LAPACK's dgetrf documents
+    the format."""
+    perm = np.asarray(perm)
+    n = perm.shape[-1]
+    flat = perm.reshape(-1, n)
+    out = np.empty_like(flat, dtype=np.int32)
+    for b in range(flat.shape[0]):
+        cur = list(range(n))
+        for k in range(n):
+            p = cur.index(int(flat[b, k]), k)
+            out[b, k] = p + 1
+            cur[k], cur[p] = cur[p], cur[k]
+    return out.reshape(perm.shape)
+
+
+def lu_info(a: np.ndarray, rule: int = PIV_FIRSTMAX) -> int:
+    """LAPACK-style info of the oracle's LU of P*A: 0, or k + 1 for the first
+    exactly zero U(k,k)."""
+    lu, _ = lu_factor_f32(a, rule)
+    z = np.nonzero(np.diag(lu) == 0.0)[0]
+    return int(z[0]) + 1 if z.size else 0

@@ -289,3 +289,29 @@ def test_reference_calc_cond_num_returns_norm():
         a = t[: n * n].reshape(n, n)
         norm1 = np.abs(a.astype(np.float64)).sum(axis=0).max()
         assert abs(golden[str(n)] / norm1 - 1) < 1e-4, n
+
+
+def test_ipiv_from_perm_replays_the_permutation():
+    """Applying the interchanges in order (LAPACK laswp) to the identity
+    gives the permutation back, for random permutations and every
+    permutation of small n (brute force)."""
+    import itertools
+
+    rng = np.random.RandomState(0)
+    cases = [rng.permutation(n) for n in (1, 2, 7, 16, 32) for _ in range(20)]
+    cases += [np.array(p) for n in range(1, 6) for p in itertools.permutations(range(n))]
+    for perm in cases:
+        ipiv = oracle.ipiv_from_perm(perm)
+        rows = list(range(perm.size))
+        for k, p in enumerate(ipiv):
+            assert k <= p - 1 < perm.size
+            rows[k], rows[p - 1] = rows[p - 1], rows[k]
+        assert rows == list(perm)
+    assert oracle.ipiv_from_perm(np.arange(5)).tolist() == [1, 2, 3, 4, 5]
+
+
+def test_lu_info_zero_pivots():
+    assert oracle.lu_info(np.zeros((4, 4), np.float32)) == 1
+    a = np.array([[1, 1, 2], [1, 1, 3], [0, 0, 1]], np.float32)   # U(1,1) = 1 - 1*1 = 0
+    assert oracle.lu_info(a) == 2
+    assert oracle.lu_info(np.eye(3, dtype=np.float32)) == 0
