@@ -385,6 +385,67 @@ def bench_refine(args) -> int:
     return 0
 
 
+def bench_rocsolver(args) -> int:
+    """This library against rocSOLVER's strided-batched getrf + getri (LAPACK
+    partial pivoting) on the same device: 1M matrices per n, U[-1,1) from
+    mt19937(42), both in place; kernel time from HIP events on the launch
+    stream, inputs restored (untimed) before every launch.  value = speedup
+    at n = 32."""
+    import ctypes
+
+    import matrixinversion_amd as mia
+    from matrixinversion_amd import workloads
+    from matrixinversion_amd._native import check, cmp_lib
+
+    batch = args.batch or 1_000_000
+    stream = mia.Stream()
+    lib = cmp_lib()
+    h = ctypes.c_void_p()
+    check(lib.lubref_rocsolver_create(ctypes.byref(h), stream.handle), "rocsolver create")
+    per_n = {}
+    for n in (8, 16, 32):
+        a = workloads.uniform_matrices(42, batch, n)
+        src = mia.DeviceBuffer.from_array(a)
+        work = mia.DeviceBuffer(a.nbytes)
+        P = mia.DeviceBuffer(batch * n * 4)
+        I = mia.DeviceBuffer(batch * 4)
+        e0, e1 = mia.Event(), mia.Event()
+
+        def timed(fn):
+            out = []
+            for i in range(args.warmup + args.steps):
+                mia.device.copy_kernel(work, src, batch * n * n, stream)
+                e0.record(stream)
+                fn()
+                e1.record(stream)
+                stream.synchronize()
+                if i >= args.warmup:
+                    out.append(e0.elapsed_ms(e1))
+            return float(np.mean(out))
+
+        t_ours = timed(lambda: mia.lu_batched_inverse(work, n, batch, mia.PIVOT_PARALLEL, P, stream))
+        t_roc = timed(lambda: check(lib.lubref_rocsolver_inverse(h, work.ptr, P.ptr, I.ptr, n, batch), "rocsolver"))
+        info = I.download((batch,), np.int32)
+        per_n[n] = {"ours_ms": t_ours, "rocsolver_ms": t_roc, "speedup": t_roc / t_ours,
+                    "ours_Mmat_s": batch / t_ours / 1e3, "rocsolver_Mmat_s": batch / t_roc / 1e3,
+                    "rocsolver_info_nonzero": int((info != 0).sum())}
+        log(f"[rocsolver] n={n}: ours {t_ours:.3f} ms, rocSOLVER getrf+getri {t_roc:.3f} ms "
+            f"({t_roc / t_ours:.1f}x)")
+        for b in (src, work, P, I):
+            b.close()
+    lib.lubref_rocsolver_destroy(h)
+    out = {
+        "metric": "speedup over rocSOLVER strided-batched getrf+getri (n=32, 1M, fp32)",
+        "value": per_n[32]["speedup"], "unit": "x", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": per_n[32]["ours_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic: mt19937(42) U[-1,1) entries",
+        "config": {"workload": f"in-place batched inverse, {batch} matrices per n, pivoted", "batch": batch},
+        "detail": per_n, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,9 +457,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n-lo", type=int, default=2, help="ragged: smallest n")
     ap.add_argument("--n-hi", type=int, default=32, help="ragged: largest n")
-    ap.add_argument("--workload", default="headline", choices=["headline", "ragged", "serve", "refine"],
+    ap.add_argument("--workload", default="headline", choices=["headline", "ragged", "serve", "refine", "rocsolver"],
                     help="headline: the metric's configuration; ragged: mixed-n batch (--batch defaults to 500k); "
-                         "serve: Poisson serving sweep; refine: precision workload (100k n=32)")
+                         "serve: Poisson serving sweep; refine: precision workload (100k n=32); "
+                         "rocsolver: against rocSOLVER getrf+getri")
     ap.add_argument("--rates", default="1000,4000,16000", help="serve: request rates (req/s)")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per rate")
     ap.add_argument("--no-serve", action="store_true", help="headline: skip the short serving run")
@@ -415,6 +477,8 @@ def main() -> int:
         return bench_serve(args)
     if args.workload == "refine":
         return bench_refine(args)
+    if args.workload == "rocsolver":
+        return bench_rocsolver(args)
 
     import matrixinversion_amd as mia
     from matrixinversion_amd import workloads

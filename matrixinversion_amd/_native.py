@@ -10,6 +10,7 @@ import ctypes
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "liblubatch.so")
+CMP_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "liblubref_rocsolver.so")
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -96,6 +97,31 @@ def lib() -> ctypes.CDLL:
                 fn.argtypes = args
         _lib = l
     return _lib
+
+
+# include/lubatch_compare.h (benchmark-only rocSOLVER comparator, its own library)
+COMPARE = {
+    "lubref_rocsolver_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]),
+    "lubref_rocsolver_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "lubref_rocsolver_inverse": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_int, ctypes.c_int]),
+}
+_cmp = None
+
+
+def cmp_lib() -> ctypes.CDLL:
+    """Load the rocSOLVER comparator library (benchmarks and tests only)."""
+    global _cmp
+    if _cmp is None:
+        if not os.path.exists(CMP_LIB_PATH):
+            raise NativeError(f"{CMP_LIB_PATH} not found (built when rocSOLVER is installed)")
+        l = ctypes.CDLL(CMP_LIB_PATH)
+        for name, (res, args) in COMPARE.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _cmp = l
+    return _cmp
 
 
 def check(status: int, what: str = "") -> None:

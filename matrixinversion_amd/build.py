@@ -24,6 +24,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIBDIR = os.path.join(PKG, "_lib")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(LIBDIR, "liblubatch.so")
+CMP_LIB = os.path.join(LIBDIR, "liblubref_rocsolver.so")
 
 ARCH = "gfx950"
 SIZE_RANGES = [(1, 8), (9, 16), (17, 20), (21, 24), (25, 28), (29, 32)]
@@ -97,7 +98,21 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_comparator()
     return LIB
+
+
+def build_comparator() -> str | None:
+    """The benchmark-only rocSOLVER comparator (its own library, so that
+    liblubatch.so never loads rocSOLVER); skipped when rocSOLVER is absent."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    if not os.path.exists(os.path.join(rocm, "include", "rocsolver", "rocsolver.h")):
+        return None
+    tmp = CMP_LIB + ".tmp"
+    _run([HIPCC, *CFLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "cmp_rocsolver.cpp"),
+          f"-L{os.path.join(rocm, 'lib')}", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{os.path.join(rocm, 'lib')}"])
+    os.replace(tmp, CMP_LIB)
+    return CMP_LIB
 
 
 def main() -> None:

@@ -11,10 +11,12 @@ from tests.conftest import ROOT
 INCLUDE = os.path.join(ROOT, "include")
 
 
-def _declared_functions():
+def _declared_functions(compare: bool = False):
+    """Functions declared by include/*.h: the liblubatch ABI, or (compare=True)
+    the comparator library's header."""
     names = []
     for h in sorted(os.listdir(INCLUDE)):
-        if not h.endswith(".h"):
+        if not h.endswith(".h") or (h == "lubatch_compare.h") != compare:
             continue
         text = open(os.path.join(INCLUDE, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -37,6 +39,17 @@ def test_library_exports_every_declared_symbol():
     assert not missing, f"declared but not exported: {missing}"
     for n in _native.ABI:
         assert n in _declared_functions(), f"{n} bound by ctypes but not declared in include/"
+
+
+def test_comparator_exports_its_header():
+    from matrixinversion_amd import _native
+
+    if not os.path.exists(_native.CMP_LIB_PATH):
+        pytest.skip("rocSOLVER comparator not built (rocSOLVER absent)")
+    lib = _native.cmp_lib()
+    names = _declared_functions(compare=True)
+    assert names and all(hasattr(lib, n) for n in names)
+    assert sorted(_native.COMPARE) == names
 
 
 def test_host_side_validation_codes():
