@@ -128,8 +128,10 @@ int lu_tuning_get(signed char* layout, signed char* grab);
 
 /* ---- LAPACK-style outputs -----------------------------------------------
  * The same inversion with getrf/getri-shaped outputs: ipiv[k] (1-based) is
- * the row swapped with row k at step k, info is 0, or k + 1 when U(k,k) of
- * P*A = LU is exactly zero (the inverse then holds inf/NaN).  The
+ * the row swapped with row k at step k, info is 0, or k + 1 for the first
+ * pivot U(k,k) of P*A = LU that is exactly zero (as LAPACK) or has no finite
+ * reciprocal (inf, NaN, |U(k,k)| below ~3e-39; LAPACK would report 0 there);
+ * the inverse then holds inf/NaN.  The
  * interchanges are those of the chosen pivot rule (the reference's first-max
  * on the un-updated column: parallel_pivot/luBatchedInplace.cuh:12-44), so
  * they may differ from LAPACK partial pivoting; applied in order they give

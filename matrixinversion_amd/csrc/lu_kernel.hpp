@@ -416,28 +416,27 @@ __device__ __forceinline__ bool pivot_parallel(const ColBlock<N, L>& w, LaneInts
 //   d = S[k][k] (broadcast from lane k%L of the group), r = 1/d
 //   g_c = S[k][c] * r, the lane's own multiplier (0 in the lane owning column
 //     k, so column k is left as it is)
-//   every row i:  S[i][c] -= S[i][k] * g_c, one v_fmac_f32_dpp per element
-//     with S[i][k] broadcast (negated) from the lane owning column k; the
-//     result for row k itself is discarded:
+//   every row i != k:  S[i][c] -= S[i][k] * g_c, one v_fmac_f32_dpp per
+//     element with S[i][k] broadcast (negated) from the lane owning column k
+//     (row k's update would be discarded, so it is not computed):
 //   row k is set to its normalised values g_c
 //   column k turns into right-hand column k.  It is kept unscaled: the right
 //     part R[i][k] = -S[i][k] / d is sigma_k * S[i][k] with sigma_k = -1/d,
 //     and every later row operation commutes with that column scale; its
 //     pivot entry becomes R[k][k] / sigma_k = -1.
 // At the end (P*A)^-1 = S * diag(sigma); sigma of the lane's columns is cs[].
-// Returns the first step whose pivot is exactly zero (U(k,k) = 0 of P*A = LU,
-// LAPACK's info - 1), or N when there is none.
+// A pivot d = 0 leaves sigma = NaN (rcp(0) = inf, then the Newton step gives
+// -0 * inf = NaN), as does a pivot without a finite reciprocal (inf, NaN, or
+// below ~3e-39); pivot_failure() reads that back.
 template <int N, int L>
-__device__ __forceinline__ int gauss_jordan(ColBlock<N, L>& w, LaneFloats<N, L>& cs, int l) {
+__device__ __forceinline__ void gauss_jordan(ColBlock<N, L>& w, LaneFloats<N, L>& cs, int l) {
   using G = Geo<N, L>;
 #pragma unroll
   for (int h = 0; h < G::R; ++h) cs[h] = 1.0f;
-  int zero_at = N;
   static_for<0, N>([&](auto K) {
     constexpr int k = decltype(K)::value;
     constexpr int kl = k % L, kh = k / L;
     const float d = dpp_f32<bcast_ctrl<L, kl>()>(w[kh][k]);
-    zero_at = (d == 0.0f && zero_at == N) ? k : zero_at;
     float r = __builtin_amdgcn_rcpf(d);
     r = __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);  // one Newton step
     const bool pl = (l == kl);                              // this lane owns column k (set kh)
@@ -450,14 +449,38 @@ __device__ __forceinline__ int gauss_jordan(ColBlock<N, L>& w, LaneFloats<N, L>&
       constexpr int h = decltype(H)::value;
       // (h in the template argument keeps the call dependent, so it is only
       // instantiated for h != kh; cross blocks exist only where R > 1)
-      if constexpr (h != kh) GjBlock<N + 0 * h, L, kl>::cross(w[h], w[kh], g[h]);
+      if constexpr (h != kh) GjBlock<N + 0 * h, L, kl, k>::cross(w[h], w[kh], g[h]);
     });
-    GjBlock<N, L, kl>::self(w[kh], g[kh]);
+    GjBlock<N, L, kl, k>::self(w[kh], g[kh]);
 #pragma unroll
     for (int h = 0; h < G::R; ++h) w[h][k] = g[h];
     w[kh][k] = pl ? -1.0f : g[kh];
   });
-  return zero_at;
+}
+
+// min over the L lanes of a group (the same DPP steps as group_max_u32).
+template <int L>
+__device__ __forceinline__ uint32_t group_min_u32(uint32_t m) {
+  m = min(m, dpp_u32<0xB1>(m));
+  m = min(m, dpp_u32<0x4E>(m));
+  if constexpr (L == 16) {
+    m = min(m, dpp_u32<0x141>(m));
+    m = min(m, dpp_u32<0x140>(m));
+  }
+  return m;
+}
+
+// First elimination step whose pivot was zero or had no finite reciprocal
+// (sigma of that column is NaN), or N: LAPACK's info - 1 for a zero pivot.
+// Lane l of a group holds sigma of columns l, l+L, ...
+template <int N, int L>
+__device__ __forceinline__ int pivot_failure(const LaneFloats<N, L>& cs, int l) {
+  using G = Geo<N, L>;
+  uint32_t first = N;
+#pragma unroll
+  for (int h = G::R - 1; h >= 0; --h)
+    if (l + L * h < N && cs[h] != cs[h]) first = l + L * h;
+  return static_cast<int>(group_min_u32<L>(first));
 }
 
 // ------------------------------------------------------------ the kernel
@@ -516,9 +539,11 @@ __device__ __forceinline__ void tile_front(ColBlock<N, L>& w, LaneInts<N, L>& pr
 
 // Phases 6-7: (P*A)^-1 in registers, column q written to column perm[q] of
 // the matrix's LDS region, which then holds A^-1 (row-major, stride LD).
-// Returns gauss_jordan's first zero-pivot step (N: none) of this lane's matrix.
+// With `failure` non-null, *failure receives pivot_failure() of this lane's
+// matrix.
 template <int N, int L>
-__device__ __forceinline__ int tile_solve(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* lds, int lane) {
+__device__ __forceinline__ void tile_solve(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* lds, int lane,
+                                           int* failure = nullptr) {
   using G = Geo<N, L>;
   const int l = lane % L;
   const int g = lane / L;
@@ -526,7 +551,8 @@ __device__ __forceinline__ int tile_solve(ColBlock<N, L>& w, const LaneInts<N, L
 
   // 6. (P*A)^-1 = S * diag(cs), column q in lane q
   LaneFloats<N, L> cs;
-  const int zero_at = gauss_jordan<N, L>(w, cs, l);
+  gauss_jordan<N, L>(w, cs, l);
+  if (failure != nullptr) *failure = pivot_failure<N, L>(cs, l);
 
   // 7. column q of (P*A)^-1 is column perm[q] of A^-1
 #pragma unroll
@@ -538,7 +564,6 @@ __device__ __forceinline__ int tile_solve(ColBlock<N, L>& w, const LaneInts<N, L
     }
   }
   wave_sync();
-  return zero_at;
 }
 
 // Phase 8 for a contiguous tile: the inverses and the pivot vectors to HBM.
@@ -549,9 +574,10 @@ __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L
   using G = Geo<N, L>;
   const int l = lane % L;
   const int g = lane / L;
-  const int zero_at = tile_solve<N, L>(w, prm, lds, lane);
+  int failure = N;
+  tile_solve<N, L>(w, prm, lds, lane, info != nullptr ? &failure : nullptr);
   tile_store<N, L>(A + m0 * (N * N), lds, valid, vec, lane);
-  if (info != nullptr && l == 0 && g < valid) info[m0 + g] = zero_at < N ? zero_at + 1 : 0;
+  if (info != nullptr && l == 0 && g < valid) info[m0 + g] = failure < N ? failure + 1 : 0;
   if (piv != nullptr && g < valid) {
 #pragma unroll
     for (int h = 0; h < G::R; ++h)
