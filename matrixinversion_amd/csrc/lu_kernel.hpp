@@ -46,6 +46,8 @@ namespace lub {
 
 constexpr int kWave = 64;
 
+
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
@@ -586,14 +588,23 @@ __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L
   wave_sync();  // the next tile reuses the LDS region
 }
 
+// Waves per SIMD each plain kernel is compiled for (its VGPR budget: 2 ->
+// 256, 3 -> 168).  3 where the LDS tile leaves room for more than 8 waves per
+// CU (160 KB / LDS_FLOATS) and the kernel fits 168 VGPRs without spilling
+// (checked per instantiation with -Rpass-analysis=kernel-resource-usage:
+// n = 25..31 with the first-max rules); elsewhere 2, the LDS or the register
+// tile being the limit anyway.
+template <int N, int L, int PIV>
+constexpr int kWavesPerEu =
+    (L == 16 && N >= 25 && N <= 31 && PIV != kPivTree && 160 * 1024 / (Geo<N, L>::LDS_FLOATS * 4) > 8) ? 3 : 2;
+
 // Each workgroup (one wave) strides over tiles.  The next full tile is
 // fetched into registers once the current one has left LDS for registers
 // (after the column read), so with several tiles per workgroup its HBM reads
 // overlap this tile's Gauss-Jordan, the longest phase.
-// amdgpu_waves_per_eu(2): at most 256 VGPRs, so two waves share each SIMD
-// (the LDS tile of n = 32 allows no more than that anyway).
+// amdgpu_waves_per_eu: see kWavesPerEu.
 template <int N, int L, int PIV>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void lu_inverse_kernel(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N, L, PIV>))) void lu_inverse_kernel(
     float* __restrict__ A, int* __restrict__ piv, int* __restrict__ info, int64_t batch, int64_t ntiles, int vec) {
   using G = Geo<N, L>;
   __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
