@@ -121,9 +121,11 @@ def bench_ragged(args) -> int:
     bufs = [mia.DeviceBuffer(flat.nbytes) for _ in range(steps)]
     plan = mia.RaggedPlan(batch)
 
+    nfloats = flat.size
+
     def restore():
         for b in bufs:
-            mia.device.copy_kernel(b, pristine, flat.size, stream)
+            mia.device.copy_kernel(b, pristine, nfloats, stream)
         stream.synchronize()
 
     def timed(run_one):
@@ -193,6 +195,19 @@ def bench_ragged(args) -> int:
         for n, sel, src, work, pv in classes:
             mia.lu_batched_inverse(work[i], n, sel.size, flags, pv, stream)
 
+    # the same per-n launches on the packed batch itself: gather each class
+    # to a contiguous buffer, invert, scatter back (what a caller of the one-n
+    # entry point does with packed mixed-size data)
+    from matrixinversion_amd._native import check, lib as _lib
+    class_offs = [mia.DeviceBuffer.from_array(offsets[sel]) for n, sel, *_ in classes]
+
+    def plain_packed(i):
+        for (n, sel, src, work, pv), co in zip(classes, class_offs):
+            check(_lib().lubx_gather_f32(work[i].ptr, bufs[i].ptr, co.ptr, n * n, sel.size, stream.handle), "gather")
+            mia.lu_batched_inverse(work[i], n, sel.size, flags, pv, stream)
+            check(_lib().lubx_scatter_f32(bufs[i].ptr, work[i].ptr, co.ptr, n * n, sel.size, stream.handle),
+                  "scatter")
+
     for i in range(warmup):
         plain(i % steps)
     stream.synchronize()
@@ -212,6 +227,22 @@ def bench_ragged(args) -> int:
     restore_plain()
     t_plain_graph = timed(lambda i: plain_graphs[i].launch())
 
+    packed_graphs = []
+    for i in range(steps):
+        g = mia.Graph(stream)
+        g.begin()
+        plain_packed(i)
+        g.end()
+        packed_graphs.append(g)
+    for _ in range(max(1, warmup)):
+        for g in packed_graphs:
+            g.launch()
+    stream.synchronize()
+    restore()
+    t_packed_graph = timed(lambda i: packed_graphs[i].launch())
+    x_packed = bufs[-1].download(x_ragged.shape, np.float32)
+    packed_same = bool(np.array_equal(x_packed.view(np.uint32), x_ragged.view(np.uint32)))
+
     # the ragged result must equal the per-n plain results bit for bit
     same = True
     for n, sel, src, work, pv in classes:
@@ -221,7 +252,7 @@ def bench_ragged(args) -> int:
         pp = pv.download((sel.size, n), np.int32)
         same &= bool(np.array_equal(p_ragged[(poffs[sel][:, None] + np.arange(n)[None, :])], pp))
     log(f"[ragged] eager {t_eager.mean():.3f} ms, graph {t_graph.mean():.3f} ms, per-n plain {t_plain.mean():.3f} ms "
-        f"(graph {t_plain_graph.mean():.3f} ms); "
+        f"(graph {t_plain_graph.mean():.3f} ms; on the packed batch with gather/scatter {t_packed_graph.mean():.3f} ms); "
         f"ragged == plain bitwise: {same}; graph == eager: {graph_same}; invalid n: {bad}")
 
     algo = workloads.ragged_bytes(ns, pivots=flags != 0)
@@ -252,6 +283,9 @@ def bench_ragged(args) -> int:
                    "plain_per_n_graph_ms_mean": float(t_plain_graph.mean()),
                    "speedup_vs_plain_per_n": float(t_plain.mean()) / ms,
                    "speedup_vs_plain_per_n_graph": float(t_plain_graph.mean()) / ms,
+                   "plain_per_n_packed_graph_ms_mean": float(t_packed_graph.mean()),
+                   "speedup_vs_plain_per_n_on_packed_data": float(t_packed_graph.mean()) / ms,
+                   "plain_on_packed_equals_ragged_bitwise": packed_same,
                    "ragged_equals_plain_bitwise": same, "graph_equals_eager_bitwise": graph_same,
                    "algorithmic_bytes_per_launch": algo, "pct_hbm": 100 * achieved / HBM_PEAK_GBPS,
                    "plain_launches_per_step": len(classes)},
@@ -263,10 +297,10 @@ def bench_ragged(args) -> int:
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
-    for g in graphs + plain_graphs:
+    for g in graphs + plain_graphs + packed_graphs:
         g.close()
     plan.close()
-    return 0 if same and graph_same and bad == 0 else 3
+    return 0 if same and graph_same and packed_same and bad == 0 else 3
 
 
 def serve_sweep(rates, duration: float, budget_gb: float = 64.0, pool: int = 200_000) -> list:

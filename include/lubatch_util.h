@@ -50,6 +50,15 @@ int   lubx_graph_capture_end(void* stream, void** graph_exec);
 int   lubx_graph_launch(void* graph_exec, void* stream);
 int   lubx_graph_destroy(void* graph_exec);
 
+/* Gather / scatter of `count` n x n matrices (nn = n*n floats each) between a
+ * packed buffer (matrix i at float offset offsets[i]) and a contiguous one:
+ * gather packed -> contiguous, scatter contiguous -> packed.  Benchmark
+ * support: the work a caller of the one-n entry point does to invert a
+ * class of a mixed-size batch in place.  The contiguous side must be
+ * 16-byte aligned. */
+int   lubx_gather_f32(float* dst, const float* packed, const int64_t* offsets, int nn, int64_t count, void* stream);
+int   lubx_scatter_f32(float* packed, const float* src, const int64_t* offsets, int nn, int64_t count, void* stream);
+
 /* Plain streaming copy kernel (dwordx4, grid-stride), used to calibrate the
  * FETCH_SIZE / WRITE_SIZE counters against a known byte count. */
 int   lubx_copy_f32(float* dst, const float* src, size_t count, void* stream);

@@ -10,6 +10,7 @@
 namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));  // 4-byte aligned: packed offsets
 
 inline int rc(hipError_t e) { return e == hipSuccess ? LUB_OK : LUB_EHIP; }
 
@@ -29,6 +30,69 @@ __global__ __launch_bounds__(256) void copy_scalar_kernel(float* __restrict__ ds
                                                           size_t begin, size_t count) {
   for (size_t i = begin + blockIdx.x * size_t(256) + threadIdx.x; i < count; i += size_t(gridDim.x) * 256)
     dst[i] = src[i];
+}
+
+// Element f of the contiguous side is element f % nn of matrix f / nn, at
+// float offset offsets[f / nn] of the packed side.  Each thread moves four
+// consecutive contiguous-side floats (one dwordx4 there; the packed side is
+// only 4-byte aligned, so element-wise) with one division per four elements;
+// I is uint32_t when every index fits, else int64_t.
+template <bool TO_CONTIGUOUS, typename I>
+__global__ __launch_bounds__(256) void gather_kernel(float* __restrict__ dst, const float* __restrict__ src,
+                                                     const int64_t* __restrict__ offsets, int nn, int64_t total) {
+  const I quads = static_cast<I>((total + 3) / 4);
+  const I n = static_cast<I>(nn);
+  for (I q = blockIdx.x * I(blockDim.x) + threadIdx.x; q < quads; q += I(gridDim.x) * blockDim.x) {
+    const I f0 = 4 * q;
+    I i = f0 / n;
+    I e = f0 - i * n;
+    int64_t base = offsets[i];
+    const bool whole = int64_t(f0) + 4 <= total;
+    if (TO_CONTIGUOUS) {
+      f32x4 v;
+      if (whole && e + 4 <= n) {  // inside one matrix: one dword-aligned dwordx4
+        v = *reinterpret_cast<const f32x4u*>(src + base + e);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (e == n) { ++i; e = 0; base = (int64_t(f0) + k < total) ? offsets[i] : 0; }
+          v[k] = (int64_t(f0) + k < total) ? src[base + e] : 0.0f;
+          ++e;
+        }
+      }
+      if (whole) *reinterpret_cast<f32x4*>(dst + f0) = v;
+      else
+        for (int k = 0; int64_t(f0) + k < total; ++k) dst[f0 + k] = v[k];
+    } else {
+      f32x4 v;
+      if (whole) v = *reinterpret_cast<const f32x4*>(src + f0);
+      else
+        for (int k = 0; k < 4; ++k) v[k] = (int64_t(f0) + k < total) ? src[f0 + k] : 0.0f;
+      if (whole && e + 4 <= n) {
+        *reinterpret_cast<f32x4u*>(dst + base + e) = v;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (e == n) { ++i; e = 0; base = (int64_t(f0) + k < total) ? offsets[i] : 0; }
+          if (int64_t(f0) + k < total) dst[base + e] = v[k];
+          ++e;
+        }
+      }
+    }
+  }
+}
+
+template <bool TO_CONTIGUOUS>
+int launch_gather(float* dst, const float* src, const int64_t* offsets, int nn, int64_t count, void* stream) {
+  const int64_t total = int64_t(nn) * count;
+  const int64_t quads = (total + 3) / 4;
+  const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((quads + 255) / 256, 1 << 20));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (total + 4 < (int64_t(1) << 32))
+    gather_kernel<TO_CONTIGUOUS, uint32_t><<<blocks, 256, 0, s>>>(dst, src, offsets, nn, total);
+  else
+    gather_kernel<TO_CONTIGUOUS, int64_t><<<blocks, 256, 0, s>>>(dst, src, offsets, nn, total);
+  return hipGetLastError() == hipSuccess ? LUB_OK : LUB_EHIP;
 }
 
 }  // namespace
@@ -142,6 +206,22 @@ int lubx_graph_launch(void* graph_exec, void* stream) {
 }
 
 int lubx_graph_destroy(void* graph_exec) { return rc(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec))); }
+
+int lubx_gather_f32(float* dst, const float* packed, const int64_t* offsets, int nn, int64_t count, void* stream) {
+  if (nn < 1 || count < 0) return LUB_EINVAL_ARG;
+  if (count == 0) return LUB_OK;
+  if (dst == nullptr || packed == nullptr || offsets == nullptr) return LUB_ENULL;
+  if (reinterpret_cast<uintptr_t>(dst) & 15) return LUB_EALIGN;  // the contiguous side moves as dwordx4
+  return launch_gather<true>(dst, packed, offsets, nn, count, stream);
+}
+
+int lubx_scatter_f32(float* packed, const float* src, const int64_t* offsets, int nn, int64_t count, void* stream) {
+  if (nn < 1 || count < 0) return LUB_EINVAL_ARG;
+  if (count == 0) return LUB_OK;
+  if (packed == nullptr || src == nullptr || offsets == nullptr) return LUB_ENULL;
+  if (reinterpret_cast<uintptr_t>(src) & 15) return LUB_EALIGN;
+  return launch_gather<false>(packed, src, offsets, nn, count, stream);
+}
 
 int lubx_copy_f32(float* dst, const float* src, size_t count, void* stream) {
   if (count == 0) return LUB_OK;

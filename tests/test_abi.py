@@ -147,6 +147,18 @@ def test_server_validation_codes():
     assert lib.lu_status_string(9).decode().startswith("no room")
 
 
+def test_gather_scatter_validation_codes():
+    from matrixinversion_amd import _native
+
+    lib = _native.lib()
+    fake = 0x1000
+    assert lib.lubx_gather_f32(fake, fake, fake, 0, 4, None) == 8
+    assert lib.lubx_gather_f32(None, fake, fake, 4, 4, None) == 3
+    assert lib.lubx_gather_f32(fake + 4, fake, fake, 4, 4, None) == 7
+    assert lib.lubx_scatter_f32(fake, fake + 4, fake, 4, 4, None) == 7
+    assert lib.lubx_scatter_f32(None, None, None, 4, 0, None) == 0
+
+
 def test_pack_ragged_host_layout():
     import numpy as np
 

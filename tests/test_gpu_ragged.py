@@ -252,3 +252,35 @@ def test_ragged_plan_capacity_is_checked(gpu):
         finally:
             for b in (A, O, NS):
                 b.close()
+
+
+def test_gather_scatter_utilities(gpu):
+    """lubx_gather_f32 / lubx_scatter_f32 (the per-n baseline's data movement)
+    against numpy, for every n, at odd offsets with gaps between matrices."""
+    from matrixinversion_amd._native import check, lib
+
+    rng = np.random.RandomState(8)
+    for n in (1, 2, 3, 5, 8, 13, 31, 32):
+        count = 97
+        mats = [rng.uniform(-1, 1, (n, n)).astype(np.float32) for _ in range(count)]
+        flat, offs, _, _ = _pack_scattered(mats, rng)
+        print(f"[case] gather/scatter n={n}", flush=True)
+        P = gpu.DeviceBuffer.from_array(flat)
+        O = gpu.DeviceBuffer.from_array(offs)
+        C = gpu.DeviceBuffer(count * n * n * 4)
+        try:
+            check(lib().lubx_gather_f32(C.ptr, P.ptr, O.ptr, n * n, count, None), "gather")
+            gpu.synchronize()
+            c = C.download((count, n, n), np.float32)
+            assert np.array_equal(c, np.stack(mats))
+            C.upload(-c)
+            check(lib().lubx_scatter_f32(P.ptr, C.ptr, O.ptr, n * n, count, None), "scatter")
+            gpu.synchronize()
+            back = P.download(flat.shape, np.float32)
+            expect = flat.copy()
+            for i, o in enumerate(offs):
+                expect[o:o + n * n] = -mats[i].ravel()
+            assert np.array_equal(back, expect), "scatter wrote outside or wrong values"
+        finally:
+            for b in (P, O, C):
+                b.close()
