@@ -38,9 +38,14 @@ __global__ __launch_bounds__(1024) void residual_kernel(const float* __restrict_
     if (t < nn) {
       const int i = t / n, j = t - i * n;
       float r = 0.0f;
-      // separate multiply and add (no FMA contraction), in l order: the
-      // arithmetic of verifyInv's inner loop as written
-      for (int l = 0; l < n; ++l) r = __fadd_rn(r, __fmul_rn(a[i * (kMaxN + 1) + l], x[l * (kMaxN + 1) + j]));
+      {
+        // separate multiply and add, in l order: the arithmetic of
+        // verifyInv's inner loop as written.  HIP contracts a*b + c into an
+        // FMA by default (also through __fmul_rn / __fadd_rn), which rounds
+        // once instead of twice and flips borderline counts.
+#pragma clang fp contract(off)
+        for (int l = 0; l < n; ++l) r = r + a[i * (kMaxN + 1) + l] * x[l * (kMaxN + 1) + j];
+      }
       const float e = fabsf(r - (i == j ? 1.0f : 0.0f));
       if (!(e < thr)) atomicAnd(&all_ok, 0);  // NaN fails, as in the reference's '<' test
       // non-negative floats order like their bit patterns; NaN maps to +inf's pattern + 1

@@ -162,10 +162,12 @@ def test_well_conditioned_bounds(gpu, mode):
         assert r.max() <= 1e-4, f"n={n}: max|AX-I|={r.max():.3e}"
 
 
-def test_device_residual_matches_verify_inv(gpu):
+@pytest.mark.parametrize("n,batch", [(16, 64), (32, 4096)])
+def test_device_residual_matches_verify_inv(gpu, n, batch):
     """lu_batched_residual counts exactly what the oracle's verifyInv
-    restatement counts (same FP32 summation order), NaN included."""
-    n, batch = 16, 64
+    restatement counts (same FP32 operations and summation order), NaN
+    included; 4096 random n=32 matrices hold borderline cases that an FMA
+    would count differently."""
     a = oracle.uniform_matrices(77, batch, n)
     x, _ = oracle.lu_inverse_f32(a, oracle.PIV_FIRSTMAX)
     x[3, 2, 2] += np.float32(0.5)
