@@ -34,12 +34,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, target_s: float = 12.0) -> dict:
+def cpu_baseline(n: int, target_s: float = 12.0, gepp: bool = False) -> dict:
     """The oracle's FP32 restatement of the reference algorithm (REF pivot rule,
-    Doolittle LU, column solves), OpenMP over matrices on the host's cores,
-    timed on a bounded sample of the same workload: the first `sample`
+    Doolittle LU, column solves) -- or, for the partial-pivoting mode, its FP32
+    Gauss-Jordan with getrf's rule -- OpenMP over matrices on the host's
+    cores, timed on a bounded sample of the same workload: the first `sample`
     matrices, inverted from a pristine copy `reps` times (~target_s seconds)."""
     import oracle
+
+    def invert(work, piv, threads):
+        if gepp:
+            oracle.batch_gepp_inplace(work, piv, threads)
+        else:
+            oracle.batch_inverse_inplace(work, piv, oracle.PIV_FIRSTMAX, threads)
 
     threads = oracle.openmp_threads()
     sample = 65536
@@ -47,19 +54,20 @@ def cpu_baseline(n: int, target_s: float = 12.0) -> dict:
     piv = np.empty((sample, n), dtype=np.int32)
     work = pristine.copy()
     t0 = time.perf_counter()
-    oracle.batch_inverse_inplace(work, piv, oracle.PIV_FIRSTMAX, threads)
+    invert(work, piv, threads)
     first = time.perf_counter() - t0
     reps = max(1, int(target_s / max(first, 1e-6)))
     total = first
     for _ in range(reps - 1):
         np.copyto(work, pristine)
         t0 = time.perf_counter()
-        oracle.batch_inverse_inplace(work, piv, oracle.PIV_FIRSTMAX, threads)
+        invert(work, piv, threads)
         total += time.perf_counter() - t0
     return {"value": reps * sample / total, "unit": "matrices/s", "cores": threads, "kind": "port",
             "sample": f"first {sample} matrices of the workload (n={n}, mt19937(42) U[-1,1)) inverted {reps} times, "
-                      f"FP32 Doolittle LU + column solves with the reference's first-max pivot rule "
-                      f"(oracle/lu_oracle.c), OpenMP over matrices, {total:.1f} s"}
+                      + ("FP32 Gauss-Jordan with getrf's partial pivoting " if gepp else
+                         "FP32 Doolittle LU + column solves with the reference's first-max pivot rule ")
+                      + f"(oracle/lu_oracle.c), OpenMP over matrices, {total:.1f} s"}
 
 
 def cpu_baseline_ragged(ns_all: np.ndarray, flat: np.ndarray, offsets: np.ndarray, target_s: float = 12.0) -> dict:
@@ -487,7 +495,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--batch", type=int, default=None)
-    ap.add_argument("--pivot", default="parallel", choices=["none", "parallel", "serial", "tree"])
+    ap.add_argument("--pivot", default="parallel", choices=["none", "parallel", "serial", "tree", "gepp"],
+                    help="pivot rule; gepp is LAPACK's partial pivoting (not a reference rule)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n-lo", type=int, default=2, help="ragged: smallest n")
     ap.add_argument("--n-hi", type=int, default=32, help="ragged: largest n")
@@ -612,7 +621,7 @@ def main() -> int:
                                       "70% pivoted / 30% unpivoted dominant, 64 GB admission budget"}
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(n)
+        out["cpu_baseline"] = cpu_baseline(n, gepp=args.pivot == "gepp")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -58,6 +58,10 @@ extern "C" {
                                     tree (luBatchedInplace.cuh:12-44): left slot wins
                                     ties and, for n not a power of two, some lanes are
                                     never reduced -- bit-compatible with that kernel     */
+#define LUB_PIVOT_GEPP      0x4  /* LAPACK getrf's rule: largest |a| of the *updated* column,
+                                    first position on ties (no counterpart in the
+                                    reference, whose rules search the un-updated column);
+                                    pivots depend on the arithmetic, ipiv is getrf's    */
 #define LUB_PIVOT_MASK      0xF
 
 /* Lane layout, bits 4..5 (a tuning override; AUTO picks per n, from the

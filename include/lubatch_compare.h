@@ -23,7 +23,10 @@ int lubref_rocsolver_create(void** handle, void* stream);
 int lubref_rocsolver_destroy(void* handle);
 
 /* getrf_strided_batched + getri_strided_batched, asynchronous on the
- * handle's stream.  ipiv: batch x n ints, info: batch ints (device). */
+ * handle's stream.  ipiv: batch x n ints, info: batch ints (device).
+ * rocSOLVER reads column-major: on a row-major batch it factors A^T, so the
+ * result, read row-major, is still A^-1 ((A^T)^-1 = (A^-1)^T) while ipiv and
+ * info describe A^T.  Pass A^T to get A's own getrf interchanges. */
 int lubref_rocsolver_inverse(void* handle, float* A, int* ipiv, int* info, int n, int batch);
 
 #ifdef __cplusplus

@@ -8,14 +8,14 @@ serving driver.
 """
 from .api import (
     LAYOUT_AUTO, LAYOUT_QUAD4, LAYOUT_ROW16, MAX_N, PIVOT_NAMES, PIVOT_NONE, PIVOT_PARALLEL, PIVOT_SERIAL,
-    PIVOT_TREE, RaggedPlan, invert, invert_lapack, invert_ragged, invert_refined, lu_batched_inverse,
+    PIVOT_TREE, PIVOT_GEPP, RaggedPlan, invert, invert_lapack, invert_ragged, invert_refined, lu_batched_inverse,
     lu_batched_inverse_ragged, pack_ragged, refine, residual, tuning_get, tuning_set, version,
 )
 from .device import DeviceBuffer, Event, Graph, Stream, device_count, device_info, synchronize
 
 __all__ = [
     "LAYOUT_AUTO", "LAYOUT_QUAD4", "LAYOUT_ROW16", "MAX_N", "PIVOT_NAMES", "PIVOT_NONE", "PIVOT_PARALLEL",
-    "PIVOT_SERIAL", "PIVOT_TREE", "RaggedPlan", "invert", "invert_lapack", "invert_ragged", "invert_refined",
+    "PIVOT_SERIAL", "PIVOT_TREE", "PIVOT_GEPP", "RaggedPlan", "invert", "invert_lapack", "invert_ragged", "invert_refined",
     "lu_batched_inverse", "lu_batched_inverse_ragged", "pack_ragged", "refine", "residual", "tuning_get",
     "tuning_set", "version", "DeviceBuffer", "Event", "Graph", "Stream", "device_count", "device_info",
     "synchronize",

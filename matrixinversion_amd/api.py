@@ -21,12 +21,13 @@ PIVOT_NONE = 0x0
 PIVOT_PARALLEL = 0x1
 PIVOT_SERIAL = 0x2
 PIVOT_TREE = 0x3
+PIVOT_GEPP = 0x4  # LAPACK getrf's partial pivoting (updated column); not a reference rule
 
 LAYOUT_AUTO = 0x00
 LAYOUT_ROW16 = 0x10
 LAYOUT_QUAD4 = 0x20
 
-PIVOT_NAMES = {"none": PIVOT_NONE, "parallel": PIVOT_PARALLEL, "serial": PIVOT_SERIAL, "tree": PIVOT_TREE}
+PIVOT_NAMES = {"none": PIVOT_NONE, "parallel": PIVOT_PARALLEL, "serial": PIVOT_SERIAL, "tree": PIVOT_TREE, "gepp": PIVOT_GEPP}
 
 MAX_N = 32
 

@@ -51,7 +51,7 @@ constexpr int kWave = 64;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-enum Pivot : int { kPivNone = 0, kPivParallel = 1, kPivSerial = 2, kPivTree = 3 };
+enum Pivot : int { kPivNone = 0, kPivParallel = 1, kPivSerial = 2, kPivTree = 3, kPivGepp = 4 };
 
 // Tile geometry for n = N with L lanes per matrix (all sizes in floats).
 // Rows whose length is a multiple of 8 floats get 4 floats of padding so that
@@ -568,16 +568,15 @@ __device__ __forceinline__ void tile_solve(ColBlock<N, L>& w, const LaneInts<N, 
   wave_sync();
 }
 
-// Phase 8 for a contiguous tile: the inverses and the pivot vectors to HBM.
+// Phase 8 for a contiguous tile whose inverses are in LDS: the inverses, the
+// info words and the pivot vectors (prm[h] = perm[l + L*h]) to HBM.
 template <int N, int L>
-__device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* __restrict__ A,
-                                          int* __restrict__ piv, int* __restrict__ info, int64_t m0, int valid,
-                                          bool vec, float* lds, int lane) {
+__device__ __forceinline__ void tile_finish(const LaneInts<N, L>& prm, int failure, float* __restrict__ A,
+                                            int* __restrict__ piv, int* __restrict__ info, int64_t m0, int valid,
+                                            bool vec, float* lds, int lane) {
   using G = Geo<N, L>;
   const int l = lane % L;
   const int g = lane / L;
-  int failure = N;
-  tile_solve<N, L>(w, prm, lds, lane, info != nullptr ? &failure : nullptr);
   tile_store<N, L>(A + m0 * (N * N), lds, valid, vec, lane);
   if (info != nullptr && l == 0 && g < valid) info[m0 + g] = failure < N ? failure + 1 : 0;
   if (piv != nullptr && g < valid) {
@@ -588,6 +587,140 @@ __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L
   wave_sync();  // the next tile reuses the LDS region
 }
 
+// Phases 6-8 of the reference's pivot rules for a contiguous tile.
+template <int N, int L>
+__device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* __restrict__ A,
+                                          int* __restrict__ piv, int* __restrict__ info, int64_t m0, int valid,
+                                          bool vec, float* lds, int lane) {
+  int failure = N;
+  tile_solve<N, L>(w, prm, lds, lane, info != nullptr ? &failure : nullptr);
+  tile_finish<N, L>(prm, failure, A, piv, info, m0, valid, vec, lds, lane);
+}
+
+// --------------------------------------------- partial pivoting (GEPP)
+
+// Gauss-Jordan with LAPACK's partial pivoting for a tile staged in LDS: at
+// step k the pivot is the largest |a| of the updated column k among the rows
+// not yet pivotal, ties going to the smallest current position, and the
+// positions of rows k and p are exchanged as getrf's row swap would.  (The
+// reference's rules search the un-updated column; this one has no
+// counterpart there.)  Rows stay in their lanes: lane l of the group holds
+// rows l, l+L, ... (w[h][c] = S[l + L*h][c]).  Step k, pivot row p,
+// d = S[p][k], r = 1/d (rcp + one Newton step):
+//   every row i != p:  S[i][c] -= (S[i][k] * r) * S[p][c]   (c != k)
+//   S[p][k] = -S[p][k];  rho[p] = r;  sigma[k] = -r
+// Row p keeps its unnormalised values and column k its unscaled ones: the
+// true matrix is rho[i] S[i][c] sigma[c], and both scales commute with every
+// later row operation.  The pivot row's lane stores it in the matrix's perm
+// slot (free until the end); lanes 0..ceil(N/4)-1 of the group read it back
+// four floats each, and each update takes its pivot-row operand from the
+// owning lane by DPP broadcast (GeppBlock), one v_fmac_f32_dpp per element.
+// After the last step row perm[k] holds row k of (P A)^-1, and
+// (P A)^-1 [k][c] = A^-1 [k][perm[c]]: the rows go to LDS by position, then
+// column q to column perm[q].  On return the region holds A^-1 (row-major,
+// stride LD), perm follows the tile, prm[h] = perm[l + L*h], and *failure
+// (if non-null) the first step whose pivot has no finite reciprocal, or N.
+// A NaN in the pivot column is taken as its largest entry.
+template <int N, int L>
+__device__ __forceinline__ void tile_gepp(LaneInts<N, L>& prm, float* lds, int lane, int* failure) {
+  using G = Geo<N, L>;
+  const int l = lane % L;
+  const int g = lane / L;
+  float* const mat = lds + g * G::MS;
+  int* const perm = reinterpret_cast<int*>(lds + G::PERM) + g * 32;
+  ColBlock<N, L> w;
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) lds_row_read<N>(mat + min(l + L * h, N - 1) * G::LD, w[h]);
+  bool rem[G::R];
+  int pos[G::R];
+  float rho[G::R];
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) {
+    rem[h] = (l + L * h) < N;  // rows past n (copies of row n-1) never pivot and are not stored
+    pos[h] = l + L * h;
+    rho[h] = 1.0f;
+  }
+  float sig[N];
+  float* const prow = reinterpret_cast<float*>(perm);  // the pivot row of the step
+  constexpr int kQ = L == 16 ? 8 : 4;                  // lanes that hold it, 4 floats each
+  static_for<0, N>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    uint32_t key[G::R];
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) key[h] = abs_key(w[h][k], rem[h]);
+    uint32_t m = key[0];
+#pragma unroll
+    for (int h = 1; h < G::R; ++h) m = max(m, key[h]);
+    m = group_max_u32<L>(m);
+    uint32_t first = 0xffu;  // smallest current position holding the maximum
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) first = (rem[h] && key[h] == m) ? min(first, (uint32_t)pos[h]) : first;
+    const int pm = static_cast<int>(group_min_u32<L>(first));
+    bool pv[G::R];
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) {
+      pv[h] = rem[h] && pos[h] == pm;
+      pos[h] = pv[h] ? k : (pos[h] == k ? pm : pos[h]);
+      rem[h] = rem[h] && !pv[h];
+    }
+#pragma unroll
+    for (int h = 0; h < G::R; ++h)
+      if (pv[h]) lds_row_write<N>(prow, w[h]);
+    wave_sync();
+    float q[4];
+    {
+      const f32x4 v = *reinterpret_cast<const f32x4*>(prow + 4 * (l % kQ));
+      q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
+    }
+    const float d = dpp_f32<bcast_ctrl<L, k / 4>()>(q[k % 4]);
+    float r = __builtin_amdgcn_rcpf(d);
+    r = __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);
+    sig[k] = -r;
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) {
+      const float mlt = pv[h] ? 0.0f : w[h][k] * r;
+      GeppBlock<N, L, k>::upd(w[h], q, mlt);  // w[h][c] -= mlt * P[c], c != k
+      w[h][k] = pv[h] ? -w[h][k] : w[h][k];
+      rho[h] = pv[h] ? r : rho[h];
+    }
+  });
+  if (failure != nullptr) {
+    int f = N;
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k) f = (sig[k] != sig[k]) ? k : f;
+    *failure = f;
+  }
+  wave_sync();  // the last pivot row has been read before its slot takes perm
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) {
+    if (l + L * h < N) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) w[h][c] = (w[h][c] * sig[c]) * rho[h];
+      lds_row_write<N>(mat + pos[h] * G::LD, w[h]);
+      perm[pos[h]] = l + L * h;
+    }
+  }
+  wave_sync();
+  // column q of (P A)^-1 -> column perm[q] of A^-1
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) {
+    const int q = min(l + L * h, N - 1);
+#pragma unroll
+    for (int p = 0; p < N; ++p) w[h][p] = mat[p * G::LD + q];
+    prm[h] = perm[q] & 31;  // a permutation of 0..n-1; the mask keeps LDS indexing in range
+  }
+  wave_sync();
+#pragma unroll
+  for (int h = 0; h < G::R; ++h) {
+    if (l + L * h < N) {
+      float* const colp = mat + prm[h];
+#pragma unroll
+      for (int i = 0; i < N; ++i) colp[i * G::LD] = w[h][i];
+    }
+  }
+  wave_sync();
+}
+
 // Waves per SIMD each plain kernel is compiled for (its VGPR budget: 2 ->
 // 256, 3 -> 168).  3 where the LDS tile leaves room for more than 8 waves per
 // CU (160 KB / LDS_FLOATS) and the kernel fits 168 VGPRs without spilling
@@ -596,7 +729,8 @@ __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L
 // tile being the limit anyway.
 template <int N, int L, int PIV>
 constexpr int kWavesPerEu =
-    (L == 16 && N >= 25 && N <= 31 && PIV != kPivTree && 160 * 1024 / (Geo<N, L>::LDS_FLOATS * 4) > 8) ? 3 : 2;
+    (L == 16 && N >= 25 && N <= 31 && PIV != kPivTree && PIV != kPivGepp &&
+     160 * 1024 / (Geo<N, L>::LDS_FLOATS * 4) > 8) ? 3 : 2;
 
 // Each workgroup (one wave) strides over tiles.  The next full tile is
 // fetched into registers once the current one has left LDS for registers
@@ -622,13 +756,22 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPer
     if (have) tile_stage<N, L>(next, lds, lane);
     else tile_load_elementwise<N, L>(A + m0 * (N * N), lds, valid, lane);
     wave_sync();
-    float w[G::R][N];
     int prm[G::R];
-    tile_front<N, L, PIV>(w, prm, lds, lane);
-    const int64_t tn = t + stride;
-    have = tn < ntiles && full(tn);
-    if (have) tile_fetch<N, L>(A + tn * G::MPW * (N * N), next, lane);
-    tile_back<N, L>(w, prm, A, piv, info, m0, valid, vec != 0, lds, lane);
+    if constexpr (PIV == kPivGepp) {
+      const int64_t tn = t + stride;
+      have = tn < ntiles && full(tn);
+      if (have) tile_fetch<N, L>(A + tn * G::MPW * (N * N), next, lane);
+      int failure = N;
+      tile_gepp<N, L>(prm, lds, lane, info != nullptr ? &failure : nullptr);
+      tile_finish<N, L>(prm, failure, A, piv, info, m0, valid, vec != 0, lds, lane);
+    } else {
+      float w[G::R][N];
+      tile_front<N, L, PIV>(w, prm, lds, lane);
+      const int64_t tn = t + stride;
+      have = tn < ntiles && full(tn);
+      if (have) tile_fetch<N, L>(A + tn * G::MPW * (N * N), next, lane);
+      tile_back<N, L>(w, prm, A, piv, info, m0, valid, vec != 0, lds, lane);
+    }
   }
 }
 

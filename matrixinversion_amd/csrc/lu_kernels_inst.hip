@@ -23,6 +23,7 @@ void register_layout(lub::KernelTable* t, int layout) {
       reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivParallel>);
   t->plain[N][lub::kPivSerial][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivSerial>);
   t->plain[N][lub::kPivTree][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivTree>);
+  t->plain[N][lub::kPivGepp][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivGepp>);
 }
 
 template <int N>

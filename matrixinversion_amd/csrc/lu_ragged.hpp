@@ -262,10 +262,14 @@ __device__ __forceinline__ void ragged_tile(float* __restrict__ A, int* __restri
     const int valid = min(G::MPW, count - base);
     ragged_stage<N, L>(x, valid, lds, lane);
     wave_sync();
-    ColBlock<N, L> w;
     LaneInts<N, L> prm;
-    tile_front<N, L, PIV>(w, prm, lds, lane);
-    tile_solve<N, L>(w, prm, lds, lane);
+    if constexpr (PIV == kPivGepp) {
+      tile_gepp<N, L>(prm, lds, lane, nullptr);
+    } else {
+      ColBlock<N, L> w;
+      tile_front<N, L, PIV>(w, prm, lds, lane);
+      tile_solve<N, L>(w, prm, lds, lane);
+    }
     auto finish = [&](int b0, int nv) {
       ragged_scatter_store<N, L>(A, sl, b0, nv, lds, lane);
       if (piv != nullptr) {

@@ -39,6 +39,7 @@ const lub::KernelTable& table() {
     lub_register_ragged_1(&g_table);
     lub_register_ragged_2(&g_table);
     lub_register_ragged_3(&g_table);
+    lub_register_ragged_4(&g_table);
   });
   return g_table;
 }
@@ -135,7 +136,7 @@ int choose_layout(int n, int flags) {
 int validate(const float* A, const int* piv, int n, int64_t batch, int flags) {
   if (n < 1 || n > LUB_MAX_N) return LUB_EINVAL_N;
   if (batch < 0) return LUB_EINVAL_BATCH;
-  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || (flags & LUB_PIVOT_MASK) > LUB_PIVOT_TREE) return LUB_EFLAGS;
+  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || (flags & LUB_PIVOT_MASK) > LUB_PIVOT_GEPP) return LUB_EFLAGS;
   if ((flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_MASK) return LUB_EFLAGS;
   if ((flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_QUAD4 && n > lub::kLayoutMaxN[1]) return LUB_EFLAGS;
   if (batch == 0) return LUB_OK;

@@ -7,7 +7,7 @@
 namespace lub {
 
 constexpr int kMaxN = 32;
-constexpr int kNumPivot = 4;   // NONE, PARALLEL, SERIAL, TREE
+constexpr int kNumPivot = 5;   // NONE, PARALLEL, SERIAL, TREE, GEPP
 constexpr int kNumLayout = 2;  // 0: 16 lanes per matrix (n <= 32), 1: 4 lanes per matrix (n <= 16)
 constexpr int kLayoutLanes[kNumLayout] = {16, 4};
 constexpr int kLayoutMaxN[kNumLayout] = {32, 16};
@@ -47,3 +47,4 @@ extern "C" void lub_register_ragged_0(lub::KernelTable* t);
 extern "C" void lub_register_ragged_1(lub::KernelTable* t);
 extern "C" void lub_register_ragged_2(lub::KernelTable* t);
 extern "C" void lub_register_ragged_3(lub::KernelTable* t);
+extern "C" void lub_register_ragged_4(lub::KernelTable* t);

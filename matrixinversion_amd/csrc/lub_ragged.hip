@@ -230,7 +230,7 @@ int lu_batched_inverse_ragged_plan(lub_ragged_plan* plan, float* A, const int64_
                                    const int64_t* piv_offsets, int64_t batch, int flags, void* stream) {
   if (plan == nullptr) return LUB_ENULL;
   if (batch < 0 || batch > kMaxRaggedBatch) return LUB_EINVAL_BATCH;
-  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || (flags & LUB_PIVOT_MASK) > LUB_PIVOT_TREE ||
+  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || (flags & LUB_PIVOT_MASK) > LUB_PIVOT_GEPP ||
       (flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_MASK)
     return LUB_EFLAGS;
   if (batch == 0) return LUB_OK;

@@ -40,7 +40,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr int kRules = 4;                 // LUB_PIVOT_NONE .. LUB_PIVOT_TREE
+constexpr int kRules = 5;                 // LUB_PIVOT_NONE .. LUB_PIVOT_GEPP
 constexpr int64_t kAlign = 256;           // request blocks start on 256-byte boundaries
 
 int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -274,7 +274,7 @@ int lu_server_submit(lub_server* s, const float* src, const int* ns, int count, 
   if (s == nullptr || src == nullptr || ns == nullptr || ticket == nullptr) return LUB_ENULL;
   if (count < 1 || count > s->max_batch) return LUB_EINVAL_BATCH;
   const int rule = flags & LUB_PIVOT_MASK;
-  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || rule > LUB_PIVOT_TREE ||
+  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || rule > LUB_PIVOT_GEPP ||
       (flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_MASK)
     return LUB_EFLAGS;
   int64_t floats = 0, pivs = 0;

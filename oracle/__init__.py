@@ -25,6 +25,7 @@ _REF_PATH = os.path.join(_HERE, "_ref", "libref_verify.so")
 PIV_NONE = 0
 PIV_FIRSTMAX = 1
 PIV_TREE = 3
+PIV_GEPP = 4  # partial pivoting on the updated column (gepp_inverse_f32)
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -68,6 +69,9 @@ def _load():
         lib.orc_verify_inv.argtypes = [_c_float_p, _c_float_p, ctypes.c_int, ctypes.c_longlong,
                                        ctypes.c_float, _c_double_p]
         lib.orc_verify_inv.restype = ctypes.c_longlong
+        lib.orc_batch_gepp_f32.argtypes = [_c_float_p, _c_int_p, _c_int_p, ctypes.c_int, ctypes.c_longlong,
+                                           ctypes.c_int]
+        lib.orc_batch_gepp_f32.restype = ctypes.c_int
         lib.orc_max_rel_err.argtypes = [_c_float_p, _c_float_p, ctypes.c_int, ctypes.c_longlong]
         lib.orc_max_rel_err.restype = ctypes.c_double
         _lib = lib
@@ -137,6 +141,22 @@ def lu_inverse_f32(a: np.ndarray, rule: int = PIV_FIRSTMAX, threads: int = 0):
     return x, piv
 
 
+def gepp_inverse_f32(a: np.ndarray, threads: int = 0):
+    """FP32 Gauss-Jordan with partial pivoting on the updated column (the
+    library's LUB_PIVOT_GEPP arithmetic; lu_oracle.c:orc_gepp_inverse_f32)
+    over a batch (b, n, n).  Returns (inverses, pivots, info)."""
+    x = np.array(a, dtype=np.float32, copy=True, order="C")
+    n = x.shape[-1]
+    batch = x.size // (n * n)
+    piv = np.empty((batch, n), dtype=np.int32)
+    info = np.empty(batch, dtype=np.int32)
+    rc = _load().orc_batch_gepp_f32(x.ctypes.data_as(_c_float_p), piv.ctypes.data_as(_c_int_p),
+                                    info.ctypes.data_as(_c_int_p), n, batch, threads)
+    if rc != 0:
+        raise ValueError("orc_batch_gepp_f32 failed")
+    return x, piv, info
+
+
 def lu_factor_f32(a: np.ndarray, rule: int = PIV_FIRSTMAX):
     """Doolittle LU of P*A for one matrix in the reference's order.
     Returns (LU, perm) with unit-lower L below the diagonal and U on/above."""
@@ -162,6 +182,17 @@ def batch_inverse_inplace(x: np.ndarray, piv, rule: int, threads: int) -> None:
     rc = _load().orc_batch_inverse_f32(x.ctypes.data_as(_c_float_p), pp, n, batch, rule, threads)
     if rc != 0:
         raise ValueError("orc_batch_inverse_f32 failed")
+
+
+def batch_gepp_inplace(x: np.ndarray, piv, threads: int) -> None:
+    """In-place batched FP32 GEPP inversion (times the CPU baseline of the
+    partial-pivoting mode)."""
+    n = x.shape[-1]
+    batch = x.size // (n * n)
+    pp = piv.ctypes.data_as(_c_int_p) if piv is not None else None
+    rc = _load().orc_batch_gepp_f32(x.ctypes.data_as(_c_float_p), pp, None, n, batch, threads)
+    if rc != 0:
+        raise ValueError("orc_batch_gepp_f32 failed")
 
 
 def inverse_f64(a: np.ndarray):

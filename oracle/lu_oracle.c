@@ -31,6 +31,10 @@
  *   orc_inverse_f64       an independent FP64 Gauss elimination with partial
  *                         pivoting on the updated column (LAPACK getrf rule),
  *                         the accuracy reference for FP32 results.
+ *   orc_gepp_inverse_f32  FP32 Gauss-Jordan with that rule (no counterpart in
+ *                         the reference, whose "partial pivoting" searches
+ *                         the un-updated column): the restatement the GPU's
+ *                         LUB_PIVOT_GEPP mode is checked against.
  *   orc_mt19937_*         MT19937 (Matsumoto & Nishimura 1998, init_genrand
  *                         seeding) and the pinned float mapping
  *                         a = 2 * ((x >> 8) * 2^-24) - 1, exact in FP32.
@@ -289,6 +293,80 @@ int orc_inverse_f64(const float* A, double* X, int n, int* gepp_perm) {
             X[i * n + c] = s / M[i * n + i];
         }
     }
+    return 0;
+}
+
+/* FP32 Gauss-Jordan inverse with partial pivoting on the updated column
+ * (LAPACK getrf's rule: largest |a| among the rows not yet pivotal, ties to
+ * the smallest current position, where step k exchanges the positions of
+ * rows k and p as getrf's row swap would).  Rows are not moved:
+ *   step k, pivot row p, d = S[p][k], r = 1/d:
+ *     every row i != p:  S[i][c] = fma(-(S[i][k] * r), S[p][c], S[i][c]), c != k
+ *     S[p][k] = -S[p][k];  row scale rho[p] = r;  column scale sigma[k] = -r
+ * Row p keeps its unnormalised values and column k its unscaled ones; the
+ * scales commute with every later row operation, so the true matrix is
+ * rho[i] * S[i][c] * sigma[c].  After the last step row i = perm[k] holds row
+ * k of (P A)^-1, and (P A)^-1 [k][c] = A^-1 [k][perm[c]].  The kernel forms r
+ * as rcp + one Newton step; here it is a division (equal but for a rare last
+ * bit).  A NaN in the pivot column is taken as its largest entry.  perm (may
+ * be NULL) receives perm[p] = original row at position p.  Returns 0, or
+ * k + 1 for the first pivot without a finite reciprocal. */
+static uint32_t orc_abs_bits(float x) {
+    uint32_t u;
+    memcpy(&u, &x, sizeof(u));
+    return u & 0x7fffffffu;
+}
+
+int orc_gepp_inverse_f32(float* A, int n, int* perm_out) {
+    float S[ORC_MAXN * ORC_MAXN], rho[ORC_MAXN], sig[ORC_MAXN];
+    int pos[ORC_MAXN], rem[ORC_MAXN], perm[ORC_MAXN];
+    int info = 0;
+    if (n < 1 || n > ORC_MAXN) return -1;
+    memcpy(S, A, sizeof(float) * (size_t)n * (size_t)n);
+    for (int i = 0; i < n; ++i) { pos[i] = i; rem[i] = 1; rho[i] = 1.0f; sig[i] = 1.0f; }
+    for (int k = 0; k < n; ++k) {
+        uint32_t m = 0;
+        for (int i = 0; i < n; ++i)
+            if (rem[i] && orc_abs_bits(S[i * n + k]) > m) m = orc_abs_bits(S[i * n + k]);
+        int p = -1;
+        for (int i = 0; i < n; ++i)
+            if (rem[i] && orc_abs_bits(S[i * n + k]) == m && (p < 0 || pos[i] < pos[p])) p = i;
+        for (int i = 0; i < n; ++i)
+            if (pos[i] == k && i != p) pos[i] = pos[p];
+        pos[p] = k;
+        rem[p] = 0;
+        const float d = S[p * n + k];
+        const float r = 1.0f / d;
+        if (!isfinite(r) && !info) info = k + 1;
+        for (int i = 0; i < n; ++i) {
+            if (i == p) continue;
+            const float mlt = S[i * n + k] * r;
+            for (int c = 0; c < n; ++c)
+                if (c != k) S[i * n + c] = fmaf(-mlt, S[p * n + c], S[i * n + c]);
+        }
+        S[p * n + k] = -d;
+        rho[p] = r;
+        sig[k] = -r;
+    }
+    for (int i = 0; i < n; ++i) perm[pos[i]] = i;
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < n; ++c) A[(size_t)pos[i] * n + perm[c]] = (S[i * n + c] * sig[c]) * rho[i];
+    if (perm_out) memcpy(perm_out, perm, sizeof(int) * (size_t)n);
+    return info;
+}
+
+/* Batched form of orc_gepp_inverse_f32 (stride n*n); piv and info may be NULL. */
+int orc_batch_gepp_f32(float* A, int* piv, int* info, int n, long long batch, int threads) {
+    if (n < 1 || n > ORC_MAXN || batch < 0) return -1;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+    for (long long b = 0; b < batch; ++b) {
+        int f = orc_gepp_inverse_f32(A + (size_t)b * n * n, n, piv ? piv + (size_t)b * n : NULL);
+        if (info) info[b] = f;
+    }
+    (void)threads;
     return 0;
 }
 

@@ -65,6 +65,7 @@ def test_host_side_validation_codes():
     assert lib.lu_batched_inverse(fake, None, 4, 1, 0x31) == 4       # both layouts
     assert lib.lu_batched_inverse(fake, None, 17, 1, 0x21) == 4      # quads hold n <= 16
     assert lib.lu_batched_inverse(fake, None, 4, 1, 0x7) == 4
+    assert lib.lu_batched_inverse(fake, None, 4, 1, 0x5) == 4       # rules end at LUB_PIVOT_GEPP (4)
     assert lib.lu_batched_inverse(fake + 2, None, 4, 1, 1) == 7      # LUB_EALIGN
     assert lib.lu_batched_inverse(None, None, 4, 0, 1) == 0          # empty batch: nothing to do
     assert lib.lu_status_string(1).decode().startswith("n outside")

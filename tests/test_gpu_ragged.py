@@ -21,6 +21,7 @@ MODES = {
     "parallel": (0x1, oracle.PIV_FIRSTMAX),
     "serial": (0x2, oracle.PIV_FIRSTMAX),
     "tree": (0x3, oracle.PIV_TREE),
+    "gepp": (0x4, oracle.PIV_GEPP),
 }
 RAGGED_QUAD_MAX_N = 12
 
@@ -88,7 +89,7 @@ def _same_bits(a, b):
 
 
 @pytest.mark.parametrize("layout", [0x10, 0x20, 0x00])
-@pytest.mark.parametrize("mode", ["parallel", "none", "serial", "tree"])
+@pytest.mark.parametrize("mode", ["parallel", "none", "serial", "tree", "gepp"])
 def test_ragged_matches_plain_and_oracle(gpu, mode, layout):
     rng = np.random.RandomState(100 + layout + len(mode))
     ns = rng.randint(1, 33, size=300)
@@ -115,7 +116,10 @@ def test_ragged_matches_plain_and_oracle(gpu, mode, layout):
         pr = np.stack([p[poffs[i]:poffs[i] + n] for i in sel])
         assert _same_bits(xr, xp), f"n={n}: ragged inverse differs from the plain launch"
         assert (pr == pp).all(), f"n={n}: ragged pivots differ from the plain launch"
-        xo, po = oracle.lu_inverse_f32(a, rule)
+        if rule == oracle.PIV_GEPP:
+            xo, po, _ = oracle.gepp_inverse_f32(a)
+        else:
+            xo, po = oracle.lu_inverse_f32(a, rule)
         assert (pr == po).all(), f"n={n}: pivots differ from the oracle"
         e = acc.rel_err64(a, xr)
         bound = acc.contract_bound(a, acc.rel_err64(a, xo))

@@ -1,6 +1,7 @@
 """GPU tests of the serving engine (lu_server_*): every request's result is
 bit-identical to inverting its matrices alone (plain launches per n), under
-any interleaving of pivot rules and sizes, with and without admission
+any interleaving of pivot rules (none, the reference's, partial pivoting)
+and sizes, with and without admission
 back-pressure.  Every case prints its name before touching the GPU."""
 import numpy as np
 import pytest
@@ -41,7 +42,10 @@ def _check_request(mia, serve, srv, t, pool, start, count, flags):
         for k in range(min(count, 8)):
             n = int(ns[k])
             a = host[offs[k]:offs[k] + n * n].reshape(1, n, n)
-            assert (oracle.pivots(a[0], oracle.PIV_FIRSTMAX) == pa[k]).all()
+            if flags == 0x4:
+                assert (oracle.gepp_inverse_f32(a)[1][0] == pa[k]).all()
+            else:
+                assert (oracle.pivots(a[0], oracle.PIV_FIRSTMAX) == pa[k]).all()
 
 
 def test_mixed_requests_match_alone(gpu):
@@ -54,7 +58,7 @@ def test_mixed_requests_match_alone(gpu):
     with serve.Server(256 << 20, max_batch=1 << 16, ring_slots=1 << 17) as srv:
         for i in range(60):
             pool = pools[0] if pivot[i] else pools[1]
-            flags = 0x1 if pivot[i] else 0x0
+            flags = (0x4 if i % 4 == 0 else 0x1) if pivot[i] else 0x0  # three rules interleaved
             ptr, ns = pool.request(int(starts[i]), int(counts[i]))
             t = srv.submit(ptr, ns, flags)
             assert t is not None

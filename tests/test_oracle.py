@@ -315,3 +315,59 @@ def test_lu_info_zero_pivots():
     a = np.array([[1, 1, 2], [1, 1, 3], [0, 0, 1]], np.float32)   # U(1,1) = 1 - 1*1 = 0
     assert oracle.lu_info(a) == 2
     assert oracle.lu_info(np.eye(3, dtype=np.float32)) == 0
+
+
+# ------------------------------------------------- partial pivoting (GEPP)
+
+def test_gepp_pivots_equal_lapack_getrf():
+    """The FP32 GEPP restatement chooses LAPACK's pivots: its permutation,
+    written as getrf interchanges, equals scipy's dgetrf ipiv on random
+    inputs (no near-ties there), and the FP64 GEPP's permutation."""
+    import scipy.linalg as sla
+
+    for n in (2, 5, 16, 31, 32):
+        a = oracle.uniform_matrices(900 + n, 60, n)
+        _, piv, info = oracle.gepp_inverse_f32(a)
+        assert (info == 0).all()
+        for i in range(len(a)):
+            _, ipiv = sla.lu_factor(a[i].astype(np.float64))
+            assert (oracle.ipiv_from_perm(piv[i]) == ipiv + 1).all(), f"n={n} matrix {i}"
+            assert (oracle.inverse_f64(a[i])[1] == piv[i]).all()
+
+
+def test_gepp_inverse_against_lapack_and_closed_forms():
+    """Values: within c*n*kappa*2^-24 of the FP64 inverse; exact on
+    permutation matrices and within 2 ulp of 1/d on diagonal ones (the
+    unscaled pivot row makes the diagonal (d*r)*r); a dropped term or a
+    transposed operand breaks these."""
+    rng = np.random.RandomState(5)
+    for n in (1, 2, 3, 7, 16, 32):
+        a = oracle.uniform_matrices(950 + n, 40, n)
+        x, _, _ = oracle.gepp_inverse_f32(a)
+        for i in range(len(a)):
+            ref = np.linalg.inv(a[i].astype(np.float64))
+            k = np.linalg.cond(a[i].astype(np.float64), 1)
+            assert np.abs(x[i] - ref).max() <= 8 * n * k * 2**-24 * np.abs(ref).max(), f"n={n} matrix {i}"
+        p = np.eye(n, dtype=np.float32)[rng.permutation(n)]
+        xp, _, _ = oracle.gepp_inverse_f32(p[None])
+        assert np.array_equal(xp[0], p.T)
+        dv = (rng.uniform(0.5, 2.0, n) * np.where(rng.randint(0, 2, n) == 1, -1.0, 1.0)).astype(np.float32)
+        xd, _, _ = oracle.gepp_inverse_f32(np.diag(dv)[None])
+        assert np.array_equal(xd[0] - np.diag(np.diag(xd[0])), np.zeros((n, n), np.float32))
+        assert np.all(np.abs(np.diag(xd[0]) - np.float32(1) / dv) <= 2 * np.spacing(np.abs(np.float32(1) / dv)))
+    a = np.array([[0, 2], [3, 1]], np.float32)                      # needs the row exchange
+    x, piv, _ = oracle.gepp_inverse_f32(a[None])
+    assert piv[0].tolist() == [1, 0] and np.allclose(x[0], [[-1 / 6, 1 / 3], [1 / 2, 0]], atol=1e-7)
+
+
+def test_gepp_ties_go_to_the_first_position():
+    """Equal |a| in the pivot column: the row at the smallest current
+    position wins (LAPACK isamax after getrf's swaps)."""
+    a = np.array([[1, 1, 0], [-1, 0, 1], [1, 2, 3]], np.float32)
+    _, piv, _ = oracle.gepp_inverse_f32(a[None])
+    assert piv[0][0] == 0
+    import scipy.linalg as sla
+    _, ipiv = sla.lu_factor(a.astype(np.float64))
+    assert (oracle.ipiv_from_perm(piv[0]) == ipiv + 1).all()
+    info = oracle.gepp_inverse_f32(np.zeros((1, 3, 3), np.float32))[2]
+    assert info[0] == 1
