@@ -23,7 +23,7 @@ struct lub_ragged_plan {
   int64_t capacity = 0;  // matrices
   char* ws = nullptr;    // lub::RaggedWs layout
   int cus = 0;           // compute units of `device`
-  int resident[4] = {};  // resident waves of each pivot rule's kernel on `device`
+  int resident[lub::kNumPivot] = {};  // resident waves of each pivot rule's kernel on `device`
   int* header() const { return reinterpret_cast<int*>(ws); }
   int64_t* soffs() const { return reinterpret_cast<int64_t*>(ws + lub::RaggedWs::slots_offset()); }
   int64_t* spoffs() const { return soffs() + capacity; }
@@ -203,7 +203,7 @@ int lu_ragged_plan_create(lub_ragged_plan** plan, int64_t max_batch) {
     delete p;
     return LUB_ENODEVICE;
   }
-  for (int pv = 0; pv < 4; ++pv) {
+  for (int pv = 0; pv < lub::kNumPivot; ++pv) {
     p->resident[pv] = kt.ragged[pv] ? p->cus * resident_per_cu(kt.ragged[pv]) : 0;
     if (p->resident[pv] <= 0) {
       delete p;
