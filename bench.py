@@ -421,7 +421,10 @@ def bench_refine(args) -> int:
         "data": "synthetic: A = Q1 diag(logspace(0,k,32)) Q2^T, Q from QR of RandomState(100+k) Gaussians",
         "config": {"workload": f"refinement, {batch} x n=32, k in (2, 4, 6), pivot=parallel", "batch": batch},
         "detail": per_k,
-        "roofline": None, "cpu_baseline": None,
+        # the refinement kernel: 12 n^2 algorithmic bytes per matrix (A and X read, X written)
+        "roofline": {"bound": "hbm", "achieved": per_k[4]["refine_GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": per_k[4]["refine_GBps"] / HBM_PEAK_GBPS, "traffic": None},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     return 0

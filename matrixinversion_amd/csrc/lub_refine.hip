@@ -11,9 +11,12 @@
 // outside their n x n block, I - A X is the identity there and X (I - A X)
 // is zero there, so the padding leaves the n x n result unchanged.
 //
-// LDS per wave: A transposed and X row-major for the FP64 residual (the
-// 4 x 4 block of lane (bi, bj) reads A^T[k][4bi..] and X[k][4bj..] as one
-// 16-byte read each per k), R row-major for the MFMA B operand.
+// LDS per wave: A and X row-major (row stride 36 floats, so rows stay
+// 16-byte aligned), R row-major for the MFMA B operand.  The FP64 residual
+// block of lane (bi, bj) reads A[4bi + r][k..k+3] and X[k + t][4bj..4bj+3] as
+// 16-byte reads, eight per four values of k; each element still sums its n
+// products in k order (the padding adds exact zeros).  A full, aligned
+// n = 32 matrix arrives as 16-byte global loads and needs no zero fill.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -31,23 +34,36 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __global__ __launch_bounds__(64) void refine_kernel(const float* __restrict__ A, float* __restrict__ X, int n,
                                                     int64_t batch) {
-  __shared__ __attribute__((aligned(16))) float sAT[kPad * kLd];
+  __shared__ __attribute__((aligned(16))) float sA[kPad * kLd];
   __shared__ __attribute__((aligned(16))) float sX[kPad * kLd];
   __shared__ __attribute__((aligned(16))) float sR[kPad * kLd];
   const int lane = threadIdx.x;
   const int nn = n * n;
+  const bool whole = n == kPad && ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(X)) & 15) == 0;
   for (int64_t m = blockIdx.x; m < batch; m += gridDim.x) {
     const float* a = A + m * nn;
     float* x = X + m * nn;
-    for (int e = lane; e < kPad * kLd; e += 64) {
-      sAT[e] = 0.0f;
-      sX[e] = 0.0f;
-    }
-    __syncthreads();
-    for (int e = lane; e < nn; e += 64) {
-      const int i = e / n, j = e - i * n;
-      sAT[j * kLd + i] = a[e];
-      sX[i * kLd + j] = x[e];
+    if (whole) {
+      // chunk c = lane + 64t: floats 4c..4c+3, row c / 8, columns 4 (c % 8) ..
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int c = lane + 64 * t, i = c >> 3, j = 4 * (c & 7);
+        const f32x4 av = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a) + c);
+        const f32x4 xv = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(x) + c);
+        *reinterpret_cast<f32x4*>(sA + i * kLd + j) = av;
+        *reinterpret_cast<f32x4*>(sX + i * kLd + j) = xv;
+      }
+    } else {
+      for (int e = lane; e < kPad * kLd; e += 64) {
+        sA[e] = 0.0f;
+        sX[e] = 0.0f;
+      }
+      __syncthreads();
+      for (int e = lane; e < nn; e += 64) {
+        const int i = e / n, j = e - i * n;
+        sA[i * kLd + j] = a[e];
+        sX[i * kLd + j] = x[e];
+      }
     }
     __syncthreads();
 
@@ -58,13 +74,19 @@ __global__ __launch_bounds__(64) void refine_kernel(const float* __restrict__ A,
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
-    for (int k = 0; k < n; ++k) {
-      const f32x4 av = *reinterpret_cast<const f32x4*>(sAT + k * kLd + 4 * bi);
-      const f32x4 xv = *reinterpret_cast<const f32x4*>(sX + k * kLd + 4 * bj);
+    const int n4 = (n + 3) & ~3;  // columns past n are zero padding
+    for (int k0 = 0; k0 < n4; k0 += 4) {
+      f32x4 av[4], xv[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < 4; ++r) av[r] = *reinterpret_cast<const f32x4*>(sA + (4 * bi + r) * kLd + k0);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = __fma_rn(double(av[r]), double(xv[c]), acc[r][c]);
+      for (int t = 0; t < 4; ++t) xv[t] = *reinterpret_cast<const f32x4*>(sX + (k0 + t) * kLd + 4 * bj);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[r][c] = __fma_rn(double(av[r][t]), double(xv[t][c]), acc[r][c]);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
