@@ -105,3 +105,30 @@ def test_nopivot_sweep_full_size(gpu, n):
     finally:
         for b in (A, X):
             b.close()
+
+
+def test_batch_beyond_int32(gpu):
+    """2^31 + 5 matrices of n = 1 (8.6 GB): the int64 batch path of
+    lu_batched_inverse_async64.  The reference's int32 offsets stop at
+    batch * n^2 < 2^31.  Every byte is 0x40, so every matrix is
+    [[3.0039062]]; the first, middle and last matrices (and the tail past
+    2^31) must hold its FP32 reciprocal, to 1 ulp, and nothing past the
+    batch may be touched."""
+    batch = 2**31 + 5
+    a = np.frombuffer(bytes([0x40] * 4), np.float32)[0]
+    expect = np.float32(1) / a
+    print(f"[case] n=1 batch={batch} (int64 path)", flush=True)
+    X = gpu.DeviceBuffer((batch + 64) * 4)
+    s = gpu.Stream()
+    try:
+        X.fill_bytes(0x40)
+        gpu.lu_batched_inverse(X, 1, batch, gpu.PIVOT_PARALLEL, None, s)
+        s.synchronize()
+        for start in (0, batch // 2 - 3, batch - 16):  # the last window spans index 2^31
+            got = X.download((16,), np.float32, offset=start * 4)
+            assert np.all(np.abs(got - expect) <= np.spacing(expect)), f"matrices {start}..: {got}"
+        beyond = X.download((64,), np.float32, offset=batch * 4)
+        assert np.all(beyond == a), "the kernel wrote past the batch"
+    finally:
+        s.close()
+        X.close()
