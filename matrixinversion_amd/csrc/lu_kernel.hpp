@@ -663,9 +663,23 @@ __device__ __forceinline__ void tile_gepp(LaneInts<N, L>& prm, float* lds, int l
       pos[h] = pv[h] ? k : (pos[h] == k ? pm : pos[h]);
       rem[h] = rem[h] && !pv[h];
     }
+    // One store block for whichever of the lane's rows is the pivot: a
+    // masked store costs the LDS about as much as a full one, so selecting
+    // first (VALU) halves the stores of a two-row lane.
+    bool mine = pv[0];
 #pragma unroll
-    for (int h = 0; h < G::R; ++h)
-      if (pv[h]) lds_row_write<N>(prow, w[h]);
+    for (int h = 1; h < G::R; ++h) mine = mine || pv[h];
+    if (mine) {
+      float prw[N];
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        float v = w[0][c];
+#pragma unroll
+        for (int h = 1; h < G::R; ++h) v = pv[h] ? w[h][c] : v;
+        prw[c] = v;
+      }
+      lds_row_write<N>(prow, prw);
+    }
     wave_sync();
     float q[4];
     {
