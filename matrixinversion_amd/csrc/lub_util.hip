@@ -137,8 +137,14 @@ void* lubx_host_alloc(size_t bytes) {
 
 int lubx_host_free(void* p) { return rc(hipHostFree(p)); }
 
+// The blocking helpers below return only once the device holds the bytes.
+// hipMemset (and hipMemcpy from pageable memory) may return while the null
+// stream is still writing, and work on a non-blocking stream does not wait
+// for the null stream: without the wait a kernel launched next on such a
+// stream can read the buffer half-filled.
 int lubx_memcpy_htod(void* dst, const void* src, size_t bytes) {
-  return rc(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  int r = rc(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return r != LUB_OK ? r : rc(hipStreamSynchronize(nullptr));
 }
 
 int lubx_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
@@ -149,7 +155,10 @@ int lubx_memcpy_dtod_async(void* dst, const void* src, size_t bytes, void* strea
   return rc(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
 }
 
-int lubx_memset(void* p, int value, size_t bytes) { return rc(hipMemset(p, value, bytes)); }
+int lubx_memset(void* p, int value, size_t bytes) {
+  int r = rc(hipMemset(p, value, bytes));
+  return r != LUB_OK ? r : rc(hipStreamSynchronize(nullptr));
+}
 
 int lubx_device_sync(void) { return rc(hipDeviceSynchronize()); }
 

@@ -81,6 +81,8 @@ class DeviceBuffer:
         check(lib().lubx_memcpy_dtod_async(self.ptr, other.ptr, n, stream.handle if stream else None), "memcpy D2D")
 
     def fill_bytes(self, value: int) -> None:
+        """Set every byte; returns once the device memory holds the value, so
+        work enqueued next on any stream sees it."""
         check(lib().lubx_memset(self.ptr, value, self.nbytes), "memset")
 
     def close(self) -> None:
