@@ -494,8 +494,8 @@ def bench_rocsolver(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--batch", type=int, default=None)
     ap.add_argument("--pivot", default="parallel", choices=["none", "parallel", "serial", "tree", "gepp"],
@@ -550,11 +550,12 @@ def main() -> int:
             mia.device.copy_kernel(b, pristine, batch * n * n, stream)
         stream.synchronize()
 
-    # warmup
+    # warmup: back-to-back launches, as in the timed region (the device needs a
+    # few tens of milliseconds of sustained load before its launch times settle)
+    restore(bufs)
     for i in range(warmup):
-        restore([bufs[i % steps]])
         mia.lu_batched_inverse(bufs[i % steps], n, batch, flags, piv, stream)
-        stream.synchronize()
+    stream.synchronize()
     restore(bufs)
 
     ev = [mia.Event() for _ in range(steps + 1)]

@@ -323,11 +323,42 @@ __device__ __forceinline__ void pivot_serial(const ColBlock<N, L>& w, LaneInts<N
   });
 }
 
+// Slot T of the tree below after levels 0..I (level i has stride n >> (i+1):
+// n/2, n/4, ..., 1).  Level i writes only slots t < n >> (i+1) and reads only
+// slots below n >> i, all written by level i-1, so each slot after level I is
+// the left-biased comparison of two slots after level I-1 -- a binary tree,
+// evaluated here depth first so that only about log2(n) slots are live.
+template <int N, int K, int T, int I>
+__device__ __forceinline__ void tree_slot(const float* col, float vk, float& v, int& p) {
+  if constexpr (I < 0) {
+    v = vk;
+    p = K;
+    if constexpr (K + 1 + T < N) {
+      const float c = col[K + 1 + T];
+      if (c > vk) { v = c; p = K + 1 + T; }
+    }
+  } else {
+    float v2;
+    int p2;
+    tree_slot<N, K, T, I - 1>(col, vk, v, p);
+    tree_slot<N, K, T + (N >> (I + 1)), I - 1>(col, vk, v2, p2);
+    if (v < v2) { v = v2; p = p2; }
+  }
+}
+
+// Number of levels of the tree for n slots (strides n/2, ..., 1).
+constexpr int tree_levels(int n) {
+  int d = 0;
+  while ((n >> (d + 1)) > 0) ++d;
+  return d;
+}
+
 // parallel_pivot's find_pivot_parallel (luBatchedInplace.cuh:12-44) with
 // threadsPerMatrix = n, evaluated redundantly by every lane: slot t starts at
 // (|a_kk|, k) and takes position k+1+t when strictly larger; the tree then
 // walks stride = n/2, n/4, ..., 1 keeping the left slot unless it is strictly
 // smaller.  For n not a power of two some slots never reach slot 0, as there.
+// Exact for ties and NaN (the comparisons are the reference's).
 template <int N, int L>
 __device__ __forceinline__ void pivot_tree(const ColBlock<N, L>& w, LaneInts<N, L>& pos, float* col,
                                            int l) {
@@ -341,23 +372,9 @@ __device__ __forceinline__ void pivot_tree(const ColBlock<N, L>& w, LaneInts<N, 
       if (l + L * h < N && pos[h] >= k) col[pos[h]] = fabsf(w[h][k]);
     wave_sync();
     const float vk = col[k];
-    float sv[N];
-    int si[N];
-#pragma unroll
-    for (int t = 0; t < N; ++t) {
-      sv[t] = vk;
-      si[t] = k;
-      if (k + 1 + t < N) {
-        const float v = col[k + 1 + t];
-        if (v > vk) { sv[t] = v; si[t] = k + 1 + t; }
-      }
-    }
-#pragma unroll
-    for (int stride = N / 2; stride > 0; stride >>= 1)
-#pragma unroll
-      for (int t = 0; t < stride; ++t)
-        if (sv[t] < sv[t + stride]) { sv[t] = sv[t + stride]; si[t] = si[t + stride]; }
-    const int best = si[0];
+    float bv;
+    int best;
+    tree_slot<N, k, 0, tree_levels(N) - 1>(col, vk, bv, best);
     wave_sync();
 #pragma unroll
     for (int h = 0; h < G::R; ++h) pos[h] = (pos[h] == k) ? best : ((pos[h] == best) ? k : pos[h]);
@@ -407,6 +424,79 @@ __device__ __forceinline__ bool pivot_parallel(const ColBlock<N, L>& w, LaneInts
   constexpr uint32_t full = (N == 32) ? 0xffffffffu : ((1u << N) - 1u);
   const bool ok = (bits == full) && (top <= 0x7f800000u);
   return __builtin_amdgcn_ballot_w64(!ok) == 0;
+}
+
+// Slots of find_pivot_parallel's tree (threadsPerMatrix = n) whose value can
+// reach slot 0: at stride s = n/2, n/4, ..., 1 slot t < s absorbs slot t + s
+// (parallel_pivot/luBatchedInplace.cuh:34-42).  All n slots for a power of
+// two; for other n some never do (n = 6: slots 0, 1, 3, 4).
+constexpr uint32_t tree_reach(int n) {
+  uint32_t from[32] = {};
+  for (int t = 0; t < n; ++t) from[t] = 1u << t;
+  for (int s = n / 2; s > 0; s >>= 1)
+    for (int t = 0; t < s; ++t) from[t] |= from[t + s];
+  return from[0];
+}
+
+// The tree rule without the tree.  Slot t starts as (|a_kk|, k) and takes
+// (|a_p|, p), p = k+1+t, only when strictly larger, and each tree comparison
+// keeps the left slot unless it is strictly smaller.  So when the candidates
+// -- position k and the positions k+1+t of slots that reach slot 0 -- hold no
+// NaN and no exact tie for their largest |a|, that strict maximum survives
+// every comparison on its way to slot 0, and the winner is the candidate with
+// the largest |a|: the DPP max of pivot_parallel over the candidates only.
+// The candidate set depends on where rows sit, so positions follow the
+// reference's swap (:165-172): the winner goes to k, the row at k to the
+// winner's position.  A power of two makes every remaining row a candidate,
+// and this is pivot_parallel.  A tie or a NaN is detected as there (two rows
+// placed at one position, or a key above +inf); the caller then reruns the
+// exact tree for the wave.
+template <int N, int L>
+__device__ __forceinline__ bool pivot_tree_dpp(const ColBlock<N, L>& w, LaneInts<N, L>& pos, int l) {
+  constexpr uint32_t full = (N == 32) ? 0xffffffffu : ((1u << N) - 1u);
+  constexpr uint32_t reach = tree_reach(N);
+  if constexpr (reach == full) {
+    return pivot_parallel<N, L>(w, pos, l);
+  } else {
+    using G = Geo<N, L>;
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) pos[h] = l + L * h;  // rows past n sit at positions >= n: never candidates
+    uint32_t top = 0;
+    static_for<0, N>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      // bit p set: position p is a candidate at step k (placed rows sit below k)
+      constexpr uint32_t cand_bits = static_cast<uint32_t>((((uint64_t)reach << 1) | 1u) << k) & full;
+      bool cand[G::R];
+      uint32_t key[G::R];
+#pragma unroll
+      for (int h = 0; h < G::R; ++h) {
+        cand[h] = ((cand_bits >> pos[h]) & 1u) != 0;
+        key[h] = abs_key(w[h][k], cand[h]);
+      }
+      uint32_t m = key[0];
+#pragma unroll
+      for (int h = 1; h < G::R; ++h) m = max(m, key[h]);
+      m = group_max_u32<L>(m);
+      top = max(top, m);
+      keep_in_vgpr(top);
+      bool win[G::R];
+      uint32_t wp = 0;
+#pragma unroll
+      for (int h = 0; h < G::R; ++h) {
+        win[h] = cand[h] && key[h] == m;
+        wp = max(wp, win[h] ? static_cast<uint32_t>(pos[h]) : 0u);
+      }
+      wp = group_max_u32<L>(wp);
+#pragma unroll
+      for (int h = 0; h < G::R; ++h) pos[h] = win[h] ? k : (pos[h] == k ? static_cast<int>(wp) : pos[h]);
+    });
+    uint32_t bits = 0;
+#pragma unroll
+    for (int h = 0; h < G::R; ++h) bits |= ((l + L * h) < N) ? (1u << pos[h]) : 0u;
+    bits = group_or_u32<L>(bits);
+    const bool ok = (bits == full) && (top <= 0x7f800000u);
+    return __builtin_amdgcn_ballot_w64(!ok) == 0;
+  }
 }
 
 // ------------------------------------------------------ Gauss-Jordan
@@ -513,8 +603,10 @@ __device__ __forceinline__ void tile_front(ColBlock<N, L>& w, LaneInts<N, L>& pr
       wave_sync();
       pivot_serial<N, L>(w, pos, mat, l);
     } else {
-      wave_sync();
-      pivot_tree<N, L>(w, pos, mat, l);
+      if (!pivot_tree_dpp<N, L>(w, pos, l)) {
+        wave_sync();
+        pivot_tree<N, L>(w, pos, mat, l);
+      }
     }
     wave_sync();
     // 4. P*A: row rho goes to position pos; perm[pos] = rho
@@ -739,11 +831,11 @@ __device__ __forceinline__ void tile_gepp(LaneInts<N, L>& prm, float* lds, int l
 // 256, 3 -> 168).  3 where the LDS tile leaves room for more than 8 waves per
 // CU (160 KB / LDS_FLOATS) and the kernel fits 168 VGPRs without spilling
 // (checked per instantiation with -Rpass-analysis=kernel-resource-usage:
-// n = 25..31 with the first-max rules); elsewhere 2, the LDS or the register
-// tile being the limit anyway.
+// n = 25..31 with the first-max rules, 25..30 with the tree rule); elsewhere
+// 2, the LDS or the register tile being the limit anyway.
 template <int N, int L, int PIV>
 constexpr int kWavesPerEu =
-    (L == 16 && N >= 25 && N <= 31 && PIV != kPivTree && PIV != kPivGepp &&
+    (L == 16 && N >= 25 && N <= 31 && PIV != kPivGepp && !(PIV == kPivTree && N == 31) &&
      160 * 1024 / (Geo<N, L>::LDS_FLOATS * 4) > 8) ? 3 : 2;
 
 // Each workgroup (one wave) strides over tiles.  The next full tile is

@@ -136,6 +136,42 @@ def test_tree_equals_firstmax_pow2_without_ties():
             assert (oracle.pivots(a, oracle.PIV_TREE) == oracle.pivots(a, oracle.PIV_FIRSTMAX)).all()
 
 
+def _tree_reach(n):
+    """Slots whose value can flow into slot 0 under strides n//2, n//4, ..., 1."""
+    frm = [{t} for t in range(n)]
+    s = n // 2
+    while s > 0:
+        for t in range(s):
+            frm[t] |= frm[t + s]
+        s //= 2
+    return frm[0]
+
+
+def test_tree_is_max_over_reachable_slots_without_ties():
+    """The closed form the GPU fast path for the tree rule relies on: with no
+    tie and no NaN among the candidates (position k, and k+1+t for every slot t
+    that reaches slot 0), the tree picks the candidate of largest |a|, and the
+    row at k swaps with it.  Slot sets by hand: n = 6 keeps {0, 1, 3, 4} (slot 2
+    absorbs 5 at stride 3 and is never merged again), n = 12 drops {2, 5, 8, 11}
+    as in the hand-derived case above, powers of two keep every slot."""
+    assert _tree_reach(6) == {0, 1, 3, 4}
+    assert _tree_reach(12) == set(range(12)) - {2, 5, 8, 11}
+    assert all(_tree_reach(n) == set(range(n)) for n in (1, 2, 4, 8, 16, 32))
+    rng = np.random.RandomState(9)
+    for n in range(3, 33):
+        reach = _tree_reach(n)
+        for _ in range(8):
+            a = rng.uniform(-1, 1, size=(n, n)).astype(np.float32)
+            at = list(range(n))  # at[p] = original row now at position p
+            for k in range(n):
+                cands = [k] + [k + 1 + t for t in sorted(reach) if k + 1 + t < n]
+                vals = [abs(a[at[p], k]) for p in cands]
+                assert len(set(vals)) == len(vals)  # U[-1,1] FP32 draws: no ties here
+                best = cands[int(np.argmax(vals))]
+                at[k], at[best] = at[best], at[k]
+            assert (oracle.pivots(a, oracle.PIV_TREE) == np.array(at)).all(), f"n={n}"
+
+
 def test_seed42_batch_tree_equals_firstmax():
     """Published: seed-42 U[-1,1] n=32 x 1000 has no exact ties, so the two
     rules agree on all 1000."""
