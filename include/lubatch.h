@@ -53,7 +53,10 @@ extern "C" {
 #define LUB_PIVOT_PARALLEL  0x1  /* REF rule, first maximum |a| wins ties, cross-lane
                                     DPP argmax (parallel_pivot/ technique, serial_pivot/
                                     tie rule); default                                   */
-#define LUB_PIVOT_SERIAL    0x2  /* REF rule, first-max, lane-serial scan (serial_pivot/) */
+#define LUB_PIVOT_SERIAL    0x2  /* REF rule, first-max (serial_pivot/'s rule): the same
+                                    pivots as LUB_PIVOT_PARALLEL and the same kernel,
+                                    whose exact fallback is serial_pivot's lane-serial
+                                    scan                                                 */
 #define LUB_PIVOT_TREE      0x3  /* REF rule with parallel_pivot/'s exact shared-memory
                                     tree (luBatchedInplace.cuh:12-44): left slot wins
                                     ties and, for n not a power of two, some lanes are

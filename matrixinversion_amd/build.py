@@ -75,7 +75,7 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
         obj = os.path.join(OBJDIR, f"lu_kernels_{lo}_{hi}.o")
         units.append([HIPCC, *CFLAGS, f"-DLUB_N_LO={lo}", f"-DLUB_N_HI={hi}", "-c",
                       os.path.join(CSRC, "lu_kernels_inst.hip"), "-o", obj])
-    for piv in range(5):
+    for piv in (0, 1, 3, 4):  # the serial rule (2) runs the first-max kernel (1)
         units.append([HIPCC, *CFLAGS, f"-DLUB_RAGGED_PIV={piv}", "-c", os.path.join(CSRC, "lu_ragged_inst.hip"),
                       "-o", os.path.join(OBJDIR, f"lu_ragged_{piv}.o")])
     for name in ("lub_api", "lub_util", "lub_verify", "lub_ragged", "lub_serve", "lub_refine", "lub_lapack"):

@@ -17,9 +17,15 @@
 //                      PARALLEL  DPP max of |a| bit patterns per column; ties
 //                                and NaN detected once per tile and resolved
 //                                by the SERIAL scan for the wave
-//                      SERIAL    first-max scan over positions (strict >,
-//                                serial_pivot/luBatchedInplace.cuh:22-36)
-//                      TREE      parallel_pivot's shared-memory tree, exactly
+//                      SERIAL    serial_pivot/'s first-max scan over positions
+//                                (strict >, serial_pivot/luBatchedInplace.cuh:
+//                                22-36): the same pivots as PARALLEL, which runs
+//                                it as its exact fallback; both flags share
+//                                one kernel
+//                      TREE      parallel_pivot's shared-memory tree, exactly:
+//                                the DPP max over the positions whose tree
+//                                slots reach slot 0; ties and NaN rerun the
+//                                tree itself for the wave
 //   4. permute       rows written back to LDS at their pivot positions: P*A
 //   5. column read   lane q holds columns q, q+L, ... of P*A
 //   6. Gauss-Jordan  row operations on P*A, pivot-free (the permutation
@@ -594,14 +600,11 @@ __device__ __forceinline__ void tile_front(ColBlock<N, L>& w, LaneInts<N, L>& pr
     for (int h = 0; h < G::R; ++h) lds_row_read<N>(mat + min(l + L * h, N - 1) * G::LD, w[h]);
     // 3. pivot positions of this lane's rows
     int pos[G::R];
-    if constexpr (PIV == kPivParallel) {
+    if constexpr (PIV == kPivParallel || PIV == kPivSerial) {  // one rule: first max
       if (!pivot_parallel<N, L>(w, pos, l)) {
         wave_sync();
         pivot_serial<N, L>(w, pos, mat, l);
       }
-    } else if constexpr (PIV == kPivSerial) {
-      wave_sync();
-      pivot_serial<N, L>(w, pos, mat, l);
     } else {
       if (!pivot_tree_dpp<N, L>(w, pos, l)) {
         wave_sync();

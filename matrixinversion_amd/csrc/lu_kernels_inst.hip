@@ -21,7 +21,10 @@ void register_layout(lub::KernelTable* t, int layout) {
   t->plain[N][lub::kPivNone][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivNone>);
   t->plain[N][lub::kPivParallel][layout] =
       reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivParallel>);
-  t->plain[N][lub::kPivSerial][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivSerial>);
+  // The serial rule's pivots are the first-max rule's: the same kernel (DPP
+  // argmax, with the lane-serial scan on ties and NaN) serves both flags.
+  t->plain[N][lub::kPivSerial][layout] =
+      reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivParallel>);
   t->plain[N][lub::kPivTree][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivTree>);
   t->plain[N][lub::kPivGepp][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivGepp>);
 }

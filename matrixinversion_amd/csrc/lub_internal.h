@@ -45,6 +45,5 @@ LUB_REGISTER_DECL(25, 28);
 LUB_REGISTER_DECL(29, 32);
 extern "C" void lub_register_ragged_0(lub::KernelTable* t);
 extern "C" void lub_register_ragged_1(lub::KernelTable* t);
-extern "C" void lub_register_ragged_2(lub::KernelTable* t);
 extern "C" void lub_register_ragged_3(lub::KernelTable* t);
 extern "C" void lub_register_ragged_4(lub::KernelTable* t);
