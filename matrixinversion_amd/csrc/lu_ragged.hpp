@@ -8,17 +8,16 @@
 // on one stream without host synchronisation or allocation (so the sequence
 // can be captured once in a hipGraph and replayed):
 //
-//   clear     zero the counters of the workspace (a kernel, not a memset:
-//             memset nodes are kept out of captured graphs)
-//   hist      matrices per n (block-private LDS histograms, one global
-//             atomic per class per block)
-//   scan      the class table: size classes ordered largest n first; per
-//             class its first slot, its first descriptor, its lane layout and
-//             the matrices per descriptor (MPW(n) * grab(n) <= 64: MPW is 4
-//             with 16-lane rows, 16 with quads)
+//   hist      matrices per n in each block of 1024 (LDS histograms), one
+//             row of counts per block, no global atomics
+//   scan      one workgroup: the class table -- size classes ordered largest
+//             n first; per class its first slot, its first descriptor, its
+//             lane layout and the matrices per descriptor (MPW(n) * grab(n)
+//             <= 64: MPW is 4 with 16-lane rows, 16 with quads) -- and each
+//             block's first slot in each class
 //   scatter   each matrix's offset and pivot offset into its class's slot
-//             range (ranks from LDS atomics, one global atomic per class per
-//             block), so a descriptor's matrices sit in consecutive slots
+//             range (its block's first slot + a rank from an LDS atomic), so
+//             a descriptor's matrices sit in consecutive slots
 //   invert    one workgroup (one wave) per descriptor, classes largest first
 //             so the tail of the launch is made of the cheapest work.  A
 //             descriptor is located from the class table held in lanes 0..32
@@ -43,18 +42,25 @@ struct RaggedTuning {
   int8_t grab[33];    // per n: wave tiles per descriptor (>= 1; at most 64 matrices per descriptor)
 };
 
-// Workspace: an int32 header, then two int64 slot arrays (offsets and pivot
-// offsets in class order).
+// Workspace: an int32 header; per planning block (CHUNK consecutive
+// matrices) 33 int32 -- its count of matrices with n outside 1..32 and per
+// class its count, which the scan turns into the block's first slot in that
+// class; then two int64 slot arrays (offsets and pivot offsets in class
+// order).  Every field is written before it is read in each call, so nothing
+// needs clearing between calls.
 struct RaggedWs {
   static constexpr int COUNTS = 0;        // [n] matrices of size n
-  static constexpr int CURSOR = 64;       // [n] slots handed out (scatter)
   static constexpr int CLASS_FIRST = 128; // [n] first slot of class n
   static constexpr int DESC_FIRST = 192;  // [n] first descriptor of class n; [0] descriptors in all
   static constexpr int META = 256;        // [2] matrices with n outside 1..32
   static constexpr int LPER = 320;        // [n] layout | matrices per descriptor << 8
   static constexpr int HEADER = 384;
-  static int64_t slots_offset() { return HEADER * 4; }  // bytes, 16-aligned
-  static int64_t bytes(int64_t cap) { return slots_offset() + 2 * 8 * cap; }
+  static constexpr int CHUNK = 1024;      // matrices per planning block
+  static constexpr int BSTRIDE = 33;      // int32 per planning block
+  static int64_t blocks(int64_t cap) { return (cap + CHUNK - 1) / CHUNK; }
+  static int64_t bhist_offset() { return HEADER * 4; }  // bytes
+  static int64_t slots_offset(int64_t cap) { return (bhist_offset() + blocks(cap) * BSTRIDE * 4 + 15) / 16 * 16; }
+  static int64_t bytes(int64_t cap) { return slots_offset(cap) + 2 * 8 * cap; }
 };
 
 constexpr int kRaggedQuadMaxN = 12;

@@ -2,7 +2,7 @@
 //
 // A plan owns the workspace of the device-side planning kernels; enqueueing
 // a batch on a plan allocates nothing and synchronises nothing, so the whole
-// sequence (clear, hist, scan, scatter, invert) can be captured in a hipGraph
+// sequence (hist, scan, scatter, invert) can be captured in a hipGraph
 // once and replayed.  lu_batched_inverse_ragged keeps one plan per host
 // thread and grows it when a larger batch arrives.
 #include <hip/hip_runtime.h>
@@ -25,7 +25,8 @@ struct lub_ragged_plan {
   int cus = 0;           // compute units of `device`
   int resident[lub::kNumPivot] = {};  // resident waves of each pivot rule's kernel on `device`
   int* header() const { return reinterpret_cast<int*>(ws); }
-  int64_t* soffs() const { return reinterpret_cast<int64_t*>(ws + lub::RaggedWs::slots_offset()); }
+  int* bhist() const { return reinterpret_cast<int*>(ws + lub::RaggedWs::bhist_offset()); }
+  int64_t* soffs() const { return reinterpret_cast<int64_t*>(ws + lub::RaggedWs::slots_offset(capacity)); }
   int64_t* spoffs() const { return soffs() + capacity; }
 };
 
@@ -34,18 +35,18 @@ namespace lub {
 // Planning kernels (one translation unit: they are not templates).
 
 constexpr int kPlanThreads = 256;
-constexpr int kPlanItems = 16;  // matrices per thread: 4096 per block
-constexpr int64_t kPlanChunk = int64_t(kPlanThreads) * kPlanItems;
+constexpr int kPlanItems = RaggedWs::CHUNK / kPlanThreads;  // matrices per thread
+constexpr int kScanThreads = 1024;                           // 32 lanes per class
+static_assert(kPlanItems * kPlanThreads == RaggedWs::CHUNK, "a planning block covers one chunk");
 
-__global__ void ragged_clear(int* ws) {
-  for (int i = threadIdx.x; i < RaggedWs::HEADER; i += blockDim.x) ws[i] = 0;
-}
-
-__global__ __launch_bounds__(kPlanThreads) void ragged_hist(const int* __restrict__ ns, int64_t batch, int* ws) {
-  __shared__ int hist[34];  // [0] matrices with n outside 1..32
-  if (threadIdx.x < 34) hist[threadIdx.x] = 0;
+// Counts per n of one chunk of CHUNK matrices (LDS histogram), written as the
+// chunk's row of the block table: [0] matrices with n outside 1..32, [n].
+__global__ __launch_bounds__(kPlanThreads) void ragged_hist(const int* __restrict__ ns, int64_t batch,
+                                                            int* __restrict__ bhist) {
+  __shared__ int hist[RaggedWs::BSTRIDE];
+  if (threadIdx.x < RaggedWs::BSTRIDE) hist[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t b0 = int64_t(blockIdx.x) * kPlanChunk;
+  const int64_t b0 = int64_t(blockIdx.x) * RaggedWs::CHUNK;
 #pragma unroll
   for (int k = 0; k < kPlanItems; ++k) {
     const int64_t i = b0 + k * kPlanThreads + threadIdx.x;
@@ -55,68 +56,94 @@ __global__ __launch_bounds__(kPlanThreads) void ragged_hist(const int* __restric
     }
   }
   __syncthreads();
-  const int c = threadIdx.x;
-  if (c >= 1 && c <= 32 && hist[c] != 0) atomicAdd(&ws[RaggedWs::COUNTS + c], hist[c]);
-  if (c == 0 && hist[0] != 0) atomicAdd(&ws[RaggedWs::META + 2], hist[0]);
+  if (threadIdx.x < RaggedWs::BSTRIDE) bhist[int64_t(blockIdx.x) * RaggedWs::BSTRIDE + threadIdx.x] = hist[threadIdx.x];
 }
 
-// The class table: classes 32, 31, ..., 1 in order; a descriptor holds
-// MPW(n) * grab(n) matrices, at most kRaggedMaxPer (one slot per lane), so
-// grab is clamped to 16 with 16-lane rows and 4 with quads.
-__global__ void ragged_scan(int* ws, RaggedTuning tune) {
-  if (threadIdx.x != 0) return;
-  int first = 0, tiles = 0;
-  for (int n = 32; n >= 1; --n) {
-    const int count = ws[RaggedWs::COUNTS + n];
-    const int layout = (tune.layout[n] == 1 && n <= kRaggedQuadMaxN) ? 1 : 0;
-    const int mpw = ragged_mpw(n, layout);
-    const int per = mpw * min(max(1, (int)tune.grab[n]), kRaggedMaxPer / mpw);
-    ws[RaggedWs::CLASS_FIRST + n] = first;
-    ws[RaggedWs::DESC_FIRST + n] = tiles;
-    ws[RaggedWs::LPER + n] = layout | (per << 8);
-    first += count;
-    tiles += (count + per - 1) / per;
+// One workgroup.  Lanes 32(c-1) .. 32(c-1)+31 own class c and split the
+// blocks into 32 consecutive ranges: per range the class count, then the
+// class table (classes 32, 31, ..., 1 in order; a descriptor holds MPW(n) *
+// grab(n) matrices, at most kRaggedMaxPer -- one slot per lane -- so grab is
+// clamped to 16 with 16-lane rows and 4 with quads), then each block's
+// count in the block table is replaced by the block's first slot in class c.
+__global__ __launch_bounds__(kScanThreads) void ragged_scan(int* __restrict__ ws, int* __restrict__ bhist,
+                                                            int64_t nblocks, RaggedTuning tune) {
+  __shared__ int part[32][33];  // [class - 1][range] (padded row)
+  __shared__ int bad[32];       // per range: matrices with n outside 1..32
+  __shared__ int count[33];
+  __shared__ int first_slot[33];
+  const int c = threadIdx.x / 32 + 1;
+  const int j = threadIdx.x % 32;
+  const int64_t per = (nblocks + 31) / 32;
+  const int64_t lo = min(nblocks, j * per);
+  const int64_t hi = min(nblocks, lo + per);
+  int sum = 0, sum0 = 0;
+#pragma unroll 8
+  for (int64_t b = lo; b < hi; ++b) {
+    sum += bhist[b * RaggedWs::BSTRIDE + c];
+    if (c == 1) sum0 += bhist[b * RaggedWs::BSTRIDE];
   }
-  ws[RaggedWs::DESC_FIRST + 0] = tiles;
-  ws[RaggedWs::CLASS_FIRST + 0] = first;
-  ws[RaggedWs::LPER + 0] = 0;
+  part[c - 1][j] = sum;
+  if (c == 1) bad[j] = sum0;
+  __syncthreads();
+  int excl = 0;
+  for (int t = 0; t < j; ++t) excl += part[c - 1][t];
+  if (j == 31) count[c] = excl + sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nbad = 0;
+    for (int t = 0; t < 32; ++t) nbad += bad[t];
+    ws[RaggedWs::META + 2] = nbad;
+    int first = 0, tiles = 0;
+    for (int n = 32; n >= 1; --n) {
+      const int cnt = count[n];
+      const int layout = (tune.layout[n] == 1 && n <= kRaggedQuadMaxN) ? 1 : 0;
+      const int mpw = ragged_mpw(n, layout);
+      const int per_desc = mpw * min(max(1, (int)tune.grab[n]), kRaggedMaxPer / mpw);
+      ws[RaggedWs::COUNTS + n] = cnt;
+      ws[RaggedWs::CLASS_FIRST + n] = first;
+      ws[RaggedWs::DESC_FIRST + n] = tiles;
+      ws[RaggedWs::LPER + n] = layout | (per_desc << 8);
+      first_slot[n] = first;
+      first += cnt;
+      tiles += (cnt + per_desc - 1) / per_desc;
+    }
+    ws[RaggedWs::DESC_FIRST + 0] = tiles;
+    ws[RaggedWs::CLASS_FIRST + 0] = first;
+    ws[RaggedWs::LPER + 0] = 0;
+  }
+  __syncthreads();
+  int slot = first_slot[c] + excl;
+  for (int64_t b = lo; b < hi; ++b) {
+    const int v = bhist[b * RaggedWs::BSTRIDE + c];
+    bhist[b * RaggedWs::BSTRIDE + c] = slot;
+    slot += v;
+  }
 }
 
-// Slot of each matrix = its class's first slot + the block's base in the
-// class (one global atomic per class per block) + its rank in the block
-// (LDS atomics).  The order within a class is therefore not fixed; results
-// do not depend on it.
+// Slot of each matrix = its block's first slot in its class (from the scan)
+// + a rank from an LDS atomic.  The order within a block is therefore not
+// fixed; results do not depend on it.
 __global__ __launch_bounds__(kPlanThreads) void ragged_scatter(const int* __restrict__ ns,
                                                                const int64_t* __restrict__ offs,
                                                                const int64_t* __restrict__ poffs, int64_t batch,
-                                                               int* ws, int64_t* __restrict__ soffs,
+                                                               const int* __restrict__ bbase,
+                                                               int64_t* __restrict__ soffs,
                                                                int64_t* __restrict__ spoffs) {
-  __shared__ int hist[33];
-  __shared__ int base[33];
-  if (threadIdx.x < 33) hist[threadIdx.x] = 0;
+  __shared__ int cur[RaggedWs::BSTRIDE];
+  if (threadIdx.x < RaggedWs::BSTRIDE) cur[threadIdx.x] = bbase[int64_t(blockIdx.x) * RaggedWs::BSTRIDE + threadIdx.x];
   __syncthreads();
-  const int64_t b0 = int64_t(blockIdx.x) * kPlanChunk;
-  int n_of[kPlanItems];
-  int rank[kPlanItems];
+  const int64_t b0 = int64_t(blockIdx.x) * RaggedWs::CHUNK;
 #pragma unroll
   for (int k = 0; k < kPlanItems; ++k) {
     const int64_t i = b0 + k * kPlanThreads + threadIdx.x;
-    const int n = i < batch ? ns[i] : 0;
-    n_of[k] = (n >= 1 && n <= 32) ? n : 0;
-    rank[k] = n_of[k] ? atomicAdd(&hist[n_of[k]], 1) : 0;
-  }
-  __syncthreads();
-  const int c = threadIdx.x;
-  if (c >= 1 && c <= 32 && hist[c] != 0)
-    base[c] = ws[RaggedWs::CLASS_FIRST + c] + atomicAdd(&ws[RaggedWs::CURSOR + c], hist[c]);
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kPlanItems; ++k) {
-    if (n_of[k] == 0) continue;
-    const int64_t i = b0 + k * kPlanThreads + threadIdx.x;
-    const int pos = base[n_of[k]] + rank[k];
-    soffs[pos] = offs[i];
-    if (poffs != nullptr) spoffs[pos] = poffs[i];
+    if (i < batch) {
+      const int n = ns[i];
+      if (n >= 1 && n <= 32) {
+        const int pos = atomicAdd(&cur[n], 1);
+        soffs[pos] = offs[i];
+        if (poffs != nullptr) spoffs[pos] = poffs[i];
+      }
+    }
   }
 }
 
@@ -146,7 +173,7 @@ void plan_release(lub_ragged_plan* p) {
   p->capacity = 0;
 }
 
-unsigned plan_blocks(int64_t batch) { return static_cast<unsigned>((batch + lub::kPlanChunk - 1) / lub::kPlanChunk); }
+unsigned plan_blocks(int64_t batch) { return static_cast<unsigned>(lub::RaggedWs::blocks(batch)); }
 
 // Matrices per descriptor of class n under tuning `rt` (the scan kernel
 // computes the same).
@@ -253,10 +280,10 @@ int lu_batched_inverse_ragged_plan(lub_ragged_plan* plan, float* A, const int64_
   hipStream_t s = static_cast<hipStream_t>(stream);
   int* hdr = plan->header();
   const unsigned blocks = plan_blocks(batch);
-  lub::ragged_clear<<<1, 256, 0, s>>>(hdr);
-  lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, hdr);
-  lub::ragged_scan<<<1, 64, 0, s>>>(hdr, rt);
-  lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, hdr,
+  int* bhist = plan->bhist();
+  lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, bhist);
+  lub::ragged_scan<<<1, lub::kScanThreads, 0, s>>>(hdr, bhist, static_cast<int64_t>(blocks), rt);
+  lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, bhist,
                                                             plan->soffs(), plan->spoffs());
   int64_t* soffs = plan->soffs();
   int64_t* spoffs = piv ? plan->spoffs() : nullptr;
