@@ -32,17 +32,19 @@ struct lub_ragged_plan {
 
 namespace lub {
 
-// Planning kernels (one translation unit: they are not templates).
+// Planning kernels (one translation unit: they are not templates).  The
+// block table is class-major: bhist[c * nblocks + b] for chunk b of this call
+// (nblocks chunks of CHUNK matrices).
 
 constexpr int kPlanThreads = 256;
 constexpr int kPlanItems = RaggedWs::CHUNK / kPlanThreads;  // matrices per thread
-constexpr int kScanThreads = 1024;                           // 32 lanes per class
+constexpr int kScanKeep = 8;  // table entries per scan lane kept in registers
 static_assert(kPlanItems * kPlanThreads == RaggedWs::CHUNK, "a planning block covers one chunk");
 
-// Counts per n of one chunk of CHUNK matrices (LDS histogram), written as the
-// chunk's row of the block table: [0] matrices with n outside 1..32, [n].
+// Counts per n of one chunk (LDS histogram), written to the chunk's column
+// of the block table: [n] for n = 1..32, [0] for n outside 1..32.
 __global__ __launch_bounds__(kPlanThreads) void ragged_hist(const int* __restrict__ ns, int64_t batch,
-                                                            int* __restrict__ bhist) {
+                                                            int* __restrict__ bhist, int64_t nblocks) {
   __shared__ int hist[RaggedWs::BSTRIDE];
   if (threadIdx.x < RaggedWs::BSTRIDE) hist[threadIdx.x] = 0;
   __syncthreads();
@@ -56,81 +58,116 @@ __global__ __launch_bounds__(kPlanThreads) void ragged_hist(const int* __restric
     }
   }
   __syncthreads();
-  if (threadIdx.x < RaggedWs::BSTRIDE) bhist[int64_t(blockIdx.x) * RaggedWs::BSTRIDE + threadIdx.x] = hist[threadIdx.x];
+  if (threadIdx.x < RaggedWs::BSTRIDE) bhist[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
 }
 
-// One workgroup.  Lanes 32(c-1) .. 32(c-1)+31 own class c and split the
-// blocks into 32 consecutive ranges: per range the class count, then the
-// class table (classes 32, 31, ..., 1 in order; a descriptor holds MPW(n) *
-// grab(n) matrices, at most kRaggedMaxPer -- one slot per lane -- so grab is
-// clamped to 16 with 16-lane rows and 4 with quads), then each block's
-// count in the block table is replaced by the block's first slot in class c.
-__global__ __launch_bounds__(kScanThreads) void ragged_scan(int* __restrict__ ws, int* __restrict__ bhist,
-                                                            int64_t nblocks, RaggedTuning tune) {
-  __shared__ int part[32][33];  // [class - 1][range] (padded row)
-  __shared__ int bad[32];       // per range: matrices with n outside 1..32
-  __shared__ int count[33];
-  __shared__ int first_slot[33];
-  const int c = threadIdx.x / 32 + 1;
-  const int j = threadIdx.x % 32;
-  const int64_t per = (nblocks + 31) / 32;
-  const int64_t lo = min(nblocks, j * per);
+// Workgroup c - 1 scans class c's row of the block table: each chunk's count
+// becomes the chunk's first slot within the class (an exclusive prefix over
+// chunks), and the class total goes to the header.  Thread t owns a
+// consecutive range of chunks; the 256 range sums are scanned in LDS.
+__global__ __launch_bounds__(kPlanThreads) void ragged_scan(int* __restrict__ ws, int* __restrict__ bhist,
+                                                            int64_t nblocks) {
+  __shared__ int part[kPlanThreads];
+  const int c = blockIdx.x + 1;
+  const int t = threadIdx.x;
+  int* __restrict__ row = bhist + c * nblocks;
+  const int64_t per = (nblocks + kPlanThreads - 1) / kPlanThreads;
+  const int64_t lo = min(nblocks, t * per);
   const int64_t hi = min(nblocks, lo + per);
-  int sum = 0, sum0 = 0;
-#pragma unroll 8
-  for (int64_t b = lo; b < hi; ++b) {
-    sum += bhist[b * RaggedWs::BSTRIDE + c];
-    if (c == 1) sum0 += bhist[b * RaggedWs::BSTRIDE];
+  const bool kept = per <= kScanKeep;
+  int v[kScanKeep];
+  int sum = 0;
+  if (kept) {
+#pragma unroll
+    for (int q = 0; q < kScanKeep; ++q) v[q] = (lo + q < hi) ? row[lo + q] : 0;
+#pragma unroll
+    for (int q = 0; q < kScanKeep; ++q) sum += v[q];
+  } else {
+    for (int64_t b = lo; b < hi; ++b) sum += row[b];
   }
-  part[c - 1][j] = sum;
-  if (c == 1) bad[j] = sum0;
+  part[t] = sum;
   __syncthreads();
-  int excl = 0;
-  for (int t = 0; t < j; ++t) excl += part[c - 1][t];
-  if (j == 31) count[c] = excl + sum;
+  for (int d = 1; d < kPlanThreads; d <<= 1) {  // inclusive scan of the range sums
+    const int add = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  int slot = part[t] - sum;
+  if (t == kPlanThreads - 1) ws[RaggedWs::COUNTS + c] = part[t];
+  if (kept) {
+#pragma unroll
+    for (int q = 0; q < kScanKeep; ++q) {
+      if (lo + q < hi) row[lo + q] = slot;
+      slot += v[q];
+    }
+  } else {
+    for (int64_t b = lo; b < hi; ++b) {
+      const int cnt = row[b];
+      row[b] = slot;
+      slot += cnt;
+    }
+  }
+}
+
+// The class table from the class totals: classes 32, 31, ..., 1 in order;
+// a descriptor holds MPW(n) * grab(n) matrices, at most kRaggedMaxPer (one
+// slot per lane), so grab is clamped to 16 with 16-lane rows and 4 with
+// quads.  Every scatter block derives it (32 values); block 0 publishes it
+// in the header for the inversion kernel.
+__device__ void ragged_class_table(int* __restrict__ ws, int64_t batch, const RaggedTuning& tune, bool publish,
+                                   int* first_slot) {
+  __shared__ int count[33];
+  __shared__ int tiles_of[33];
+  __shared__ int lper[33];
+  const int c = threadIdx.x;
+  if (c >= 1 && c <= 32) {
+    const int cnt = ws[RaggedWs::COUNTS + c];
+    const int layout = (tune.layout[c] == 1 && c <= kRaggedQuadMaxN) ? 1 : 0;
+    const int mpw = ragged_mpw(c, layout);
+    const int per_desc = mpw * min(max(1, (int)tune.grab[c]), kRaggedMaxPer / mpw);
+    count[c] = cnt;
+    tiles_of[c] = (cnt + per_desc - 1) / per_desc;
+    lper[c] = layout | (per_desc << 8);
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int nbad = 0;
-    for (int t = 0; t < 32; ++t) nbad += bad[t];
-    ws[RaggedWs::META + 2] = nbad;
+  if (c == 0) {
     int first = 0, tiles = 0;
     for (int n = 32; n >= 1; --n) {
-      const int cnt = count[n];
-      const int layout = (tune.layout[n] == 1 && n <= kRaggedQuadMaxN) ? 1 : 0;
-      const int mpw = ragged_mpw(n, layout);
-      const int per_desc = mpw * min(max(1, (int)tune.grab[n]), kRaggedMaxPer / mpw);
-      ws[RaggedWs::COUNTS + n] = cnt;
-      ws[RaggedWs::CLASS_FIRST + n] = first;
-      ws[RaggedWs::DESC_FIRST + n] = tiles;
-      ws[RaggedWs::LPER + n] = layout | (per_desc << 8);
       first_slot[n] = first;
-      first += cnt;
-      tiles += (cnt + per_desc - 1) / per_desc;
+      if (publish) {
+        ws[RaggedWs::CLASS_FIRST + n] = first;
+        ws[RaggedWs::DESC_FIRST + n] = tiles;
+        ws[RaggedWs::LPER + n] = lper[n];
+      }
+      first += count[n];
+      tiles += tiles_of[n];
     }
-    ws[RaggedWs::DESC_FIRST + 0] = tiles;
-    ws[RaggedWs::CLASS_FIRST + 0] = first;
-    ws[RaggedWs::LPER + 0] = 0;
+    if (publish) {
+      ws[RaggedWs::DESC_FIRST + 0] = tiles;
+      ws[RaggedWs::CLASS_FIRST + 0] = first;
+      ws[RaggedWs::LPER + 0] = 0;
+      ws[RaggedWs::META + 2] = static_cast<int>(batch - first);  // matrices with n outside 1..32
+    }
   }
   __syncthreads();
-  int slot = first_slot[c] + excl;
-  for (int64_t b = lo; b < hi; ++b) {
-    const int v = bhist[b * RaggedWs::BSTRIDE + c];
-    bhist[b * RaggedWs::BSTRIDE + c] = slot;
-    slot += v;
-  }
 }
 
-// Slot of each matrix = its block's first slot in its class (from the scan)
-// + a rank from an LDS atomic.  The order within a block is therefore not
-// fixed; results do not depend on it.
+// Slot of each matrix = its class's first slot + its chunk's first slot
+// within the class (from the scan) + a rank from an LDS atomic.  The order
+// within a chunk is therefore not fixed; results do not depend on it.
 __global__ __launch_bounds__(kPlanThreads) void ragged_scatter(const int* __restrict__ ns,
                                                                const int64_t* __restrict__ offs,
                                                                const int64_t* __restrict__ poffs, int64_t batch,
-                                                               const int* __restrict__ bbase,
+                                                               int* __restrict__ ws, const int* __restrict__ bhist,
+                                                               int64_t nblocks, RaggedTuning tune,
                                                                int64_t* __restrict__ soffs,
                                                                int64_t* __restrict__ spoffs) {
-  __shared__ int cur[RaggedWs::BSTRIDE];
-  if (threadIdx.x < RaggedWs::BSTRIDE) cur[threadIdx.x] = bbase[int64_t(blockIdx.x) * RaggedWs::BSTRIDE + threadIdx.x];
+  __shared__ int first_slot[33];
+  __shared__ int cur[33];
+  ragged_class_table(ws, batch, tune, blockIdx.x == 0, first_slot);
+  const int c = threadIdx.x;
+  if (c >= 1 && c <= 32) cur[c] = first_slot[c] + bhist[c * nblocks + blockIdx.x];
   __syncthreads();
   const int64_t b0 = int64_t(blockIdx.x) * RaggedWs::CHUNK;
 #pragma unroll
@@ -281,10 +318,11 @@ int lu_batched_inverse_ragged_plan(lub_ragged_plan* plan, float* A, const int64_
   int* hdr = plan->header();
   const unsigned blocks = plan_blocks(batch);
   int* bhist = plan->bhist();
-  lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, bhist);
-  lub::ragged_scan<<<1, lub::kScanThreads, 0, s>>>(hdr, bhist, static_cast<int64_t>(blocks), rt);
-  lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, bhist,
-                                                            plan->soffs(), plan->spoffs());
+  const int64_t nblocks = blocks;
+  lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, bhist, nblocks);
+  lub::ragged_scan<<<32, lub::kPlanThreads, 0, s>>>(hdr, bhist, nblocks);
+  lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, hdr,
+                                                            bhist, nblocks, rt, plan->soffs(), plan->spoffs());
   int64_t* soffs = plan->soffs();
   int64_t* spoffs = piv ? plan->spoffs() : nullptr;
   const int* hdr_arg = hdr;

@@ -157,6 +157,56 @@ def test_ragged_invalid_sizes_are_skipped_and_counted(gpu):
         assert _same_bits(x[offs[i]:offs[i] + n * n].reshape(n, n), xp), f"matrix {i}"
 
 
+def test_ragged_large_batch_planning(gpu):
+    """2.2M small matrices (n = 1..3) with invalid sizes sprinkled in.  Past
+    2M matrices each lane of the planning scan walks more than 8 chunks of
+    1024 (its general path); every class must still land in its slots:
+    results and pivots equal the plain launch of each n bit for bit, the gaps
+    left by invalid entries stay untouched, and the invalid entries are
+    counted."""
+    rng = np.random.RandomState(6)
+    batch = 2_200_000
+    ns = rng.randint(1, 4, size=batch).astype(np.int32)
+    bad = rng.choice(batch, size=1000, replace=False)
+    sizes = ns.astype(np.int64) ** 2
+    offs = np.zeros(batch, np.int64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    poffs = np.zeros(batch, np.int64)
+    poffs[1:] = np.cumsum(ns[:-1].astype(np.int64))
+    flat = rng.uniform(-1, 1, size=int(sizes.sum())).astype(np.float32)
+    ns_run = ns.copy()
+    ns_run[bad] = np.where(np.arange(bad.size) % 2 == 0, 0, 40)
+    total = int(ns.sum())
+    print(f"[case] ragged large batch={batch} invalid={bad.size}", flush=True)
+    A = gpu.DeviceBuffer.from_array(flat)
+    O, NS, PO = (gpu.DeviceBuffer.from_array(v) for v in (offs, ns_run, poffs))
+    P = gpu.DeviceBuffer(total * 4)
+    with gpu.RaggedPlan(batch) as plan:
+        try:
+            P.fill_bytes(0xFF)
+            gpu.lu_batched_inverse_ragged(A, O, NS, batch, 0x1, P, PO, plan=plan)
+            gpu.synchronize()
+            assert plan.invalid_count() == bad.size
+            x = A.download(flat.shape, np.float32)
+            p = P.download((total,), np.int32)
+        finally:
+            for b in (A, O, NS, PO, P):
+                b.close()
+    skipped = np.zeros(batch, bool)
+    skipped[bad] = True
+    for n in range(1, 4):
+        sel = np.nonzero((ns == n) & ~skipped)[0]
+        idx = offs[sel][:, None] + np.arange(n * n)[None, :]
+        a = flat[idx].reshape(-1, n, n)
+        xp, pp = gpu.invert(a, 0x1)
+        assert _same_bits(x[idx].reshape(-1, n, n), xp), f"n={n}: ragged inverse differs from the plain launch"
+        pidx = poffs[sel][:, None] + np.arange(n)[None, :]
+        assert (p[pidx] == pp).all(), f"n={n}: ragged pivots differ from the plain launch"
+        gone = np.nonzero((ns == n) & skipped)[0]
+        gidx = offs[gone][:, None] + np.arange(n * n)[None, :]
+        assert _same_bits(x[gidx], flat[gidx]), f"n={n}: a skipped matrix was written"
+
+
 def test_ragged_without_pivots_and_single_class(gpu):
     rng = np.random.RandomState(4)
     for n in (1, 4, 13, 32):
