@@ -104,8 +104,9 @@ int lu_batched_inverse_async64(float* A, int* piv, int n, int64_t batch, int fla
  * and launch per MATRIXSIZE (parallel_pivot/run.py:198-234). */
 typedef struct lub_ragged_plan lub_ragged_plan;
 
-/* Workspace for batches of up to max_batch matrices (device memory of 16
- * bytes per matrix); a plan may be used by one stream at a time. */
+/* Workspace for batches of up to max_batch matrices (device memory of about
+ * 16.1 bytes per matrix: two int64 slots, and 33 int32 per chunk of 1024);
+ * a plan may be used by one stream at a time. */
 int lu_ragged_plan_create(lub_ragged_plan** plan, int64_t max_batch);
 int lu_ragged_plan_destroy(lub_ragged_plan* plan);
 
