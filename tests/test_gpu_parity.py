@@ -102,6 +102,20 @@ def test_ties_use_exact_rule(gpu):
                 assert (piv == po).all(), f"ties n={n} {mode} {layout}: pivots differ"
 
 
+def test_real_valued_ties_of_the_1m_workload(gpu):
+    """The 17 matrices of the 1M x n=32 seed-42 workload on which the tree rule
+    and first-max disagree: each has an exact |a| tie at a column maximum
+    (tests/golden/seed42_tie_matrices.npz).  The DPP fast paths must hand them
+    to the exact rule, so every flag still matches its oracle rule."""
+    import os
+
+    from tests.conftest import GOLDEN
+
+    a = np.load(os.path.join(GOLDEN, "seed42_tie_matrices.npz"))["a"]
+    for mode in ("parallel", "serial", "tree"):
+        _check(gpu, a, mode, "seed42 1M ties")
+
+
 def test_nan_and_zero_pivot_rules(gpu):
     """NaN at position k is never beaten, NaN elsewhere never wins; all-zero
     columns keep position k (strict '>' of the reference)."""

@@ -172,6 +172,28 @@ def test_tree_is_max_over_reachable_slots_without_ties():
             assert (oracle.pivots(a, oracle.PIV_TREE) == np.array(at)).all(), f"n={n}"
 
 
+def test_seed42_tie_fixture_holds_exact_ties():
+    """tests/golden/seed42_tie_matrices.npz (tests/golden/make_tie_fixture.py):
+    the matrices of the 1M x n=32 seed-42 workload where the tree rule and
+    first-max disagree.  Each disagreement is an exact tie.  At the first step
+    where the two pivot vectors differ, the rows each rule placed there hold
+    the same |a| in that column, and no remaining row holds more.  The first
+    matrix is the workload's own matrix at its index."""
+    f = np.load(os.path.join(GOLDEN, "seed42_tie_matrices.npz"))
+    idx, a = f["index"], f["a"]
+    assert a.shape == (len(idx), 32, 32) and len(idx) == 17
+    fm = oracle.batch_pivots(a, oracle.PIV_FIRSTMAX)
+    tr = oracle.batch_pivots(a, oracle.PIV_TREE)
+    for m in range(len(idx)):
+        k = int(np.nonzero(fm[m] != tr[m])[0][0])
+        col = np.abs(a[m][:, k])
+        assert col[fm[m][k]] == col[tr[m][k]], f"matrix {idx[m]}: not a tie at step {k}"
+        assert col[fm[m][k]] == col[fm[m][k:]].max(), f"matrix {idx[m]}: not a column maximum"
+    from matrixinversion_amd import workloads
+    first = workloads.uniform_matrices(42, int(idx[0]) + 1, 32)[-1]
+    assert np.array_equal(first, a[0])
+
+
 def test_seed42_batch_tree_equals_firstmax():
     """Published: seed-42 U[-1,1] n=32 x 1000 has no exact ties, so the two
     rules agree on all 1000."""
