@@ -550,13 +550,18 @@ def main() -> int:
             mia.device.copy_kernel(b, pristine, batch * n * n, stream)
         stream.synchronize()
 
-    # warmup: back-to-back launches, as in the timed region (the device needs a
-    # few tens of milliseconds of sustained load before its launch times settle)
+    # Restore the timed buffers first, then warm up on a copy of their own:
+    # back-to-back launches, as in the timed region (the device needs a few
+    # tens of milliseconds of sustained load before its launch times settle,
+    # and a long copy just before the timed region unsettles them again).
     restore(bufs)
-    for i in range(warmup):
-        mia.lu_batched_inverse(bufs[i % steps], n, batch, flags, piv, stream)
-    stream.synchronize()
-    restore(bufs)
+    if warmup > 0:
+        warm = mia.DeviceBuffer(nbytes)
+        restore([warm])
+        for i in range(warmup):
+            mia.lu_batched_inverse(warm, n, batch, flags, piv, stream)
+        stream.synchronize()
+        warm.close()
 
     ev = [mia.Event() for _ in range(steps + 1)]
     stream.synchronize()
