@@ -282,6 +282,13 @@ def bench_ragged(args) -> int:
     algo = workloads.ragged_bytes(ns, pivots=flags != 0)
     ms = float(t_graph.mean())
     achieved = algo / (ms * 1e-3) / 1e9
+    # HBM bytes per inversion launch from the rocprofv3 counter passes of this
+    # workload (FETCH_SIZE + WRITE_SIZE, calibrated by a copy of known size)
+    ragged_traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_lu_ragged_n{args.n_lo}-{args.n_hi}_{args.pivot}_{batch}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            ragged_traffic = json.load(f).get("hbm_bytes_per_launch")
     out = {
         "metric": "matrices/s (ragged mixed-n batch, pivoted)",
         "value": batch / (ms * 1e-3),
@@ -301,7 +308,7 @@ def bench_ragged(args) -> int:
                                f"int64 offsets, one hipGraph-replayed plan + inversion launch",
                    "batch": batch, "pivot": args.pivot, "global_batch": batch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": ragged_traffic},
         "detail": {"graph_ms_mean": ms, "eager_ms_mean": float(t_eager.mean()),
                    "plain_per_n_ms_mean": float(t_plain.mean()),
                    "plain_per_n_graph_ms_mean": float(t_plain_graph.mean()),
