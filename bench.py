@@ -514,6 +514,61 @@ def bench_rocsolver(args) -> int:
     return 0
 
 
+def bench_ragged_classes(args) -> int:
+    """Single-class batches through the ragged entry against the plain launch
+    of the same matrices: per n, `batch` (default 200k) matrices packed
+    contiguously, both entry points timed with HIP events over K back-to-back
+    launches after W warmup launches.  Shows which size classes the ragged
+    kernel loses on (its planning kernels are included)."""
+    import matrixinversion_amd as mia
+
+    batch = args.batch or 200_000
+    flags = mia.PIVOT_NAMES[args.pivot]
+    steps, warmup = args.steps, max(1, args.warmup)
+    stream = mia.Stream()
+    rng = np.random.RandomState(1)
+
+    def timeit(fn):
+        for _ in range(warmup):
+            fn()
+        stream.synchronize()
+        e0, e1 = mia.Event(), mia.Event()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_ms(e1) / steps
+
+    rows = {}
+    for n in (32, 31, 28, 24, 20, 17, 16, 13, 12, 9, 8, 5, 3):
+        a = rng.uniform(-1, 1, size=batch * n * n).astype(np.float32)
+        A = mia.DeviceBuffer.from_array(a)
+        O = mia.DeviceBuffer.from_array(np.arange(batch, dtype=np.int64) * n * n)
+        NS = mia.DeviceBuffer.from_array(np.full(batch, n, np.int32))
+        PO = mia.DeviceBuffer.from_array(np.arange(batch, dtype=np.int64) * n)
+        P = mia.DeviceBuffer(batch * n * 4)
+        with mia.RaggedPlan(batch) as plan:
+            t_plain = timeit(lambda: mia.lu_batched_inverse(A, n, batch, flags, P, stream))
+            t_rag = timeit(lambda: mia.lu_batched_inverse_ragged(A, O, NS, batch, flags, P, PO, plan=plan,
+                                                                  stream=stream))
+        for b in (A, O, NS, PO, P):
+            b.close()
+        rows[n] = {"plain_us": t_plain * 1e3, "ragged_us": t_rag * 1e3, "ratio": t_rag / t_plain}
+        log(f"[classes] n={n:2d} plain {t_plain * 1e3:7.1f} us ragged {t_rag * 1e3:7.1f} us ratio {t_rag / t_plain:.2f}")
+    out = {
+        "metric": "ragged / plain launch time, single-class batches", "value": rows[32]["ratio"],
+        "unit": "x (at n=32)", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": rows[32]["ragged_us"] / 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic: RandomState(1) U[-1,1), one n per batch",
+        "config": {"workload": f"single-class ragged vs plain, {batch} matrices per n, pivot={args.pivot}",
+                   "batch": batch, "pivot": args.pivot, "global_batch": batch},
+        "detail": rows, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -526,7 +581,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n-lo", type=int, default=2, help="ragged: smallest n")
     ap.add_argument("--n-hi", type=int, default=32, help="ragged: largest n")
-    ap.add_argument("--workload", default="headline", choices=["headline", "ragged", "serve", "refine", "rocsolver"],
+    ap.add_argument("--workload", default="headline", choices=["headline", "ragged", "ragged_classes", "serve", "refine", "rocsolver"],
                     help="headline: the metric's configuration; ragged: mixed-n batch (--batch defaults to 500k); "
                          "serve: Poisson serving sweep; refine: precision workload (100k n=32); "
                          "rocsolver: against rocSOLVER getrf+getri")
@@ -542,6 +597,8 @@ def main() -> int:
 
     if args.workload == "ragged":
         return bench_ragged(args)
+    if args.workload == "ragged_classes":
+        return bench_ragged_classes(args)
     if args.workload == "serve":
         return bench_serve(args)
     if args.workload == "refine":
