@@ -34,12 +34,46 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, target_s: float = 12.0, gepp: bool = False) -> dict:
-    """The oracle's FP32 restatement of the reference algorithm (REF pivot rule,
-    Doolittle LU, column solves) -- or, for the partial-pivoting mode, its FP32
-    Gauss-Jordan with getrf's rule -- OpenMP over matrices on the host's
-    cores, timed on a bounded sample of the same workload: the first `sample`
-    matrices, inverted from a pristine copy `reps` times (~target_s seconds)."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _repeat(run, reset, target_s: float):
+    """Runs `run` (after `reset`) until about target_s seconds of it were
+    timed; returns (repetitions, seconds timed)."""
+    reset()
+    t0 = time.perf_counter()
+    run()
+    total = time.perf_counter() - t0
+    reps = max(1, int(target_s / max(total, 1e-6)))
+    for _ in range(reps - 1):
+        reset()
+        t0 = time.perf_counter()
+        run()
+        total += time.perf_counter() - t0
+    return reps, total
+
+
+def cpu_baseline(n: int, target_s: float = 10.0, gepp: bool = False) -> dict:
+    """The reference's host path, timed on this host's cores on bounded samples
+    of the same workload:
+
+    * inversion: the oracle's FP32 restatement of the reference algorithm
+      (first-max pivots on the un-updated column, Doolittle LU, column
+      solves) -- or, for the partial-pivoting mode, its FP32 Gauss-Jordan with
+      getrf's rule -- with OpenMP over matrices on all cores (`value`) and on
+      one core (`value_1core`);
+    * the check: the oracle's restatement of verifyInv (parallel_pivot/
+      verify.hpp:50-103) single-threaded, as the reference builds it without
+      -fopenmp (README.md:51), and on all cores.
+    """
     import oracle
 
     def invert(work, piv, threads):
@@ -49,25 +83,31 @@ def cpu_baseline(n: int, target_s: float = 12.0, gepp: bool = False) -> dict:
             oracle.batch_inverse_inplace(work, piv, oracle.PIV_FIRSTMAX, threads)
 
     threads = oracle.openmp_threads()
-    sample = 65536
+    sample, sample1 = 65536, 8192
     pristine = oracle.uniform_matrices(42, sample, n)
     piv = np.empty((sample, n), dtype=np.int32)
     work = pristine.copy()
-    t0 = time.perf_counter()
-    invert(work, piv, threads)
-    first = time.perf_counter() - t0
-    reps = max(1, int(target_s / max(first, 1e-6)))
-    total = first
-    for _ in range(reps - 1):
-        np.copyto(work, pristine)
-        t0 = time.perf_counter()
-        invert(work, piv, threads)
-        total += time.perf_counter() - t0
+    reps, total = _repeat(lambda: invert(work, piv, threads), lambda: np.copyto(work, pristine), target_s)
+    w1 = pristine[:sample1].copy()
+    reps1, total1 = _repeat(lambda: invert(w1, piv[:sample1], 1), lambda: np.copyto(w1, pristine[:sample1]),
+                            target_s / 2)
+    x = pristine[:sample1].copy()
+    invert(x, piv[:sample1], threads)
+    a1 = pristine[:sample1]
+    vreps1, vtotal1 = _repeat(lambda: oracle.verify_inv(a1, x, 1e-3, threads=1), lambda: None, target_s / 3)
+    vreps, vtotal = _repeat(lambda: oracle.verify_inv(a1, x, 1e-3, threads=0), lambda: None, target_s / 5)
+    rule = ("FP32 Gauss-Jordan with getrf's partial pivoting" if gepp else
+            "FP32 Doolittle LU + column solves with the reference's first-max pivot rule")
     return {"value": reps * sample / total, "unit": "matrices/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample} matrices of the workload (n={n}, mt19937(42) U[-1,1)) inverted {reps} times, "
-                      + ("FP32 Gauss-Jordan with getrf's partial pivoting " if gepp else
-                         "FP32 Doolittle LU + column solves with the reference's first-max pivot rule ")
-                      + f"(oracle/lu_oracle.c), OpenMP over matrices, {total:.1f} s"}
+            "sample": f"inversion: first {sample} matrices of the workload (n={n}, mt19937(42) U[-1,1)) "
+                      f"inverted {reps} times, {rule} (oracle/lu_oracle.c), OpenMP over matrices on {threads} "
+                      f"threads, {total:.1f} s; 1 core: first {sample1}, {reps1} times, {total1:.1f} s; "
+                      f"verifyInv restatement on {sample1} inverses: 1 thread {vreps1} times ({vtotal1:.1f} s), "
+                      f"{threads} threads {vreps} times ({vtotal:.1f} s)",
+            "value_1core": reps1 * sample1 / total1,
+            "verify_inv_1thread_matrices_per_s": vreps1 * sample1 / vtotal1,
+            "verify_inv_all_cores_matrices_per_s": vreps * sample1 / vtotal,
+            "cpu_model": cpu_model(), "host_logical_cpus": os.cpu_count()}
 
 
 def cpu_baseline_ragged(ns_all: np.ndarray, flat: np.ndarray, offsets: np.ndarray, target_s: float = 12.0) -> dict:
@@ -101,22 +141,20 @@ def cpu_baseline_ragged(ns_all: np.ndarray, flat: np.ndarray, offsets: np.ndarra
                       f"{total:.1f} s"}
 
 
-def bench_ragged(args) -> int:
-    """Mixed-n batch (500k matrices, n ~ U{2..32}, pivoted, packed with int64
-    offsets): the ragged entry (plan + persistent kernel) eager and replayed
-    from a hipGraph, against one plain launch per n over the same matrices
+def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 2, n_hi: int = 32) -> dict:
+    """Mixed-n batch (`batch` matrices, n ~ U{n_lo..n_hi}, packed with int64
+    offsets): the ragged entry (plan + inversion kernel) eager and replayed
+    from a hipGraph, against one plain launch per n (a) over the same matrices
     pre-grouped into per-n contiguous buffers (the plain path's best case: no
-    gather cost counted)."""
+    gather cost counted) and (b) on the packed batch itself (gather each class,
+    invert, scatter back).  Returns the measurements; `ok` is False when any
+    result differs bitwise."""
     import matrixinversion_amd as mia
     from matrixinversion_amd import workloads
 
-    batch = args.batch or 500_000
-    flags = mia.PIVOT_NAMES[args.pivot]
-    steps, warmup = args.steps, args.warmup
-    info = mia.device_info(0)
-    log(f"[ragged] device {info.arch} CUs={info.cu_count}")
+    flags = mia.PIVOT_NAMES[pivot]
     t0 = time.perf_counter()
-    flat, offsets, ns, poffs = workloads.ragged_batch(42, batch, args.n_lo, args.n_hi)
+    flat, offsets, ns, poffs = workloads.ragged_batch(42, batch, n_lo, n_hi)
     log(f"[ragged] generated {batch} matrices ({flat.nbytes / 1e6:.0f} MB) in {time.perf_counter() - t0:.1f}s; "
         f"E[n]={ns.mean():.2f} E[n^2]={(ns.astype(np.int64) ** 2).mean():.1f}")
     total_piv = int(ns.sum())
@@ -149,8 +187,7 @@ def bench_ragged(args) -> int:
 
     def timed(run_one):
         """K steps back to back between one event pair (events between graph
-        launches would add their own cost): the mean step time, repeated as
-        an array for the summary statistics."""
+        launches would add their own cost): the mean step time."""
         e0, e1 = mia.Event(), mia.Event()
         stream.synchronize()
         e0.record(stream)
@@ -158,10 +195,24 @@ def bench_ragged(args) -> int:
             run_one(i)
         e1.record(stream)
         stream.synchronize()
-        return np.full(steps, e0.elapsed_ms(e1) / steps)
+        return e0.elapsed_ms(e1) / steps
 
     def ragged(i):
         mia.lu_batched_inverse_ragged(bufs[i], O, NS, batch, flags, P, PO, plan=plan, stream=stream)
+
+    def graphs_of(fn):
+        out = []
+        for i in range(steps):
+            g = mia.Graph(stream)
+            g.begin()
+            fn(i)
+            g.end()
+            out.append(g)
+        for _ in range(max(1, warmup)):  # every graph's first launch is slower: keep it out of the timing
+            for g in out:
+                g.launch()
+        stream.synchronize()
+        return out
 
     # eager ragged
     for i in range(warmup):
@@ -176,17 +227,7 @@ def bench_ragged(args) -> int:
     bad = int(plan.invalid_count())
 
     # graph-replayed ragged: one graph per resident copy
-    graphs = []
-    for i in range(steps):
-        g = mia.Graph(stream)
-        g.begin()
-        ragged(i)
-        g.end()
-        graphs.append(g)
-    for _ in range(max(1, warmup)):  # every graph's first launch is slower: keep it out of the timing
-        for g in graphs:
-            g.launch()
-    stream.synchronize()
+    graphs = graphs_of(ragged)
     restore()
     settle()
     t_graph = timed(lambda i: graphs[i].launch())
@@ -235,32 +276,12 @@ def bench_ragged(args) -> int:
     restore_plain()
     settle()
     t_plain = timed(plain)
-    plain_graphs = []
-    for i in range(steps):
-        g = mia.Graph(stream)
-        g.begin()
-        plain(i)
-        g.end()
-        plain_graphs.append(g)
-    for _ in range(max(1, warmup)):
-        for g in plain_graphs:
-            g.launch()
-    stream.synchronize()
+    plain_graphs = graphs_of(plain)
     restore_plain()
     settle()
     t_plain_graph = timed(lambda i: plain_graphs[i].launch())
 
-    packed_graphs = []
-    for i in range(steps):
-        g = mia.Graph(stream)
-        g.begin()
-        plain_packed(i)
-        g.end()
-        packed_graphs.append(g)
-    for _ in range(max(1, warmup)):
-        for g in packed_graphs:
-            g.launch()
-    stream.synchronize()
+    packed_graphs = graphs_of(plain_packed)
     restore()
     settle()
     t_packed_graph = timed(lambda i: packed_graphs[i].launch())
@@ -275,27 +296,57 @@ def bench_ragged(args) -> int:
         same &= bool(np.array_equal(x_ragged[idx].view(np.uint32), xp.view(np.uint32)))
         pp = pv.download((sel.size, n), np.int32)
         same &= bool(np.array_equal(p_ragged[(poffs[sel][:, None] + np.arange(n)[None, :])], pp))
-    log(f"[ragged] eager {t_eager.mean():.3f} ms, graph {t_graph.mean():.3f} ms, per-n plain {t_plain.mean():.3f} ms "
-        f"(graph {t_plain_graph.mean():.3f} ms; on the packed batch with gather/scatter {t_packed_graph.mean():.3f} ms); "
+    log(f"[ragged] eager {t_eager:.3f} ms, graph {t_graph:.3f} ms, per-n plain {t_plain:.3f} ms "
+        f"(graph {t_plain_graph:.3f} ms; on the packed batch with gather/scatter {t_packed_graph:.3f} ms); "
         f"ragged == plain bitwise: {same}; graph == eager: {graph_same}; invalid n: {bad}")
 
+    for g in graphs + plain_graphs + packed_graphs:
+        g.close()
+    plan.close()
+    for b in bufs + [pristine, O, NS, PO, P, wbuf] + class_offs:
+        b.close()
+    for n, sel, src, work, pv in classes:
+        for b in [src, pv] + work:
+            b.close()
+    stream.close()
     algo = workloads.ragged_bytes(ns, pivots=flags != 0)
-    ms = float(t_graph.mean())
+    return {"batch": batch, "graph_ms_mean": t_graph, "eager_ms_mean": t_eager,
+            "plain_per_n_ms_mean": t_plain, "plain_per_n_graph_ms_mean": t_plain_graph,
+            "speedup_vs_plain_per_n": t_plain / t_graph,
+            "speedup_vs_plain_per_n_graph": t_plain_graph / t_graph,
+            "plain_per_n_packed_graph_ms_mean": t_packed_graph,
+            "speedup_vs_plain_per_n_on_packed_data": t_packed_graph / t_graph,
+            "plain_on_packed_equals_ragged_bitwise": packed_same,
+            "ragged_equals_plain_bitwise": same, "graph_equals_eager_bitwise": graph_same,
+            "invalid_n": bad, "algorithmic_bytes_per_launch": algo,
+            "pct_hbm": 100 * algo / (t_graph * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+            "plain_launches_per_step": len(classes),
+            "ok": same and graph_same and packed_same and bad == 0,
+            "baselines": "plain_per_n: one lu_batched_inverse per n on matrices pre-grouped by n outside the timed "
+                         "region; plain_per_n_packed: per n, gather the class from the packed batch, invert, "
+                         "scatter back (graph-captured)"}
+
+
+def bench_ragged(args) -> int:
+    """The ragged workload as its own bench line (value = matrices/s of the
+    graph-replayed ragged launch)."""
+    import matrixinversion_amd as mia
+
+    batch = args.batch or 500_000
+    info = mia.device_info(0)
+    log(f"[ragged] device {info.arch} CUs={info.cu_count}")
+    d = ragged_measure(batch, args.pivot, args.steps, args.warmup, args.n_lo, args.n_hi)
+    ms = d["graph_ms_mean"]
+    algo = d["algorithmic_bytes_per_launch"]
     achieved = algo / (ms * 1e-3) / 1e9
-    # HBM bytes per inversion launch from the rocprofv3 counter passes of this
-    # workload (FETCH_SIZE + WRITE_SIZE, calibrated by a copy of known size)
-    ragged_traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_lu_ragged_n{args.n_lo}-{args.n_hi}_{args.pivot}_{batch}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            ragged_traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = load_traffic(f"traffic_lu_ragged_n{args.n_lo}-{args.n_hi}_{args.pivot}_{batch}.json")
     out = {
         "metric": "matrices/s (ragged mixed-n batch, pivoted)",
         "value": batch / (ms * 1e-3),
         "unit": "matrices/s",
         "n_gpus": 1,
-        "steps": steps,
-        "warmup": warmup,
+        "steps": args.steps,
+        "warmup": args.warmup,
         "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
@@ -308,30 +359,34 @@ def bench_ragged(args) -> int:
                                f"int64 offsets, one hipGraph-replayed plan + inversion launch",
                    "batch": batch, "pivot": args.pivot, "global_batch": batch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": ragged_traffic},
-        "detail": {"graph_ms_mean": ms, "eager_ms_mean": float(t_eager.mean()),
-                   "plain_per_n_ms_mean": float(t_plain.mean()),
-                   "plain_per_n_graph_ms_mean": float(t_plain_graph.mean()),
-                   "speedup_vs_plain_per_n": float(t_plain.mean()) / ms,
-                   "speedup_vs_plain_per_n_graph": float(t_plain_graph.mean()) / ms,
-                   "plain_per_n_packed_graph_ms_mean": float(t_packed_graph.mean()),
-                   "speedup_vs_plain_per_n_on_packed_data": float(t_packed_graph.mean()) / ms,
-                   "plain_on_packed_equals_ragged_bitwise": packed_same,
-                   "ragged_equals_plain_bitwise": same, "graph_equals_eager_bitwise": graph_same,
-                   "algorithmic_bytes_per_launch": algo, "pct_hbm": 100 * achieved / HBM_PEAK_GBPS,
-                   "plain_launches_per_step": len(classes)},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic["bytes"],
+                     "traffic_build_id": traffic["build_id"], "traffic_stale": traffic["stale"]},
+        "detail": d,
     }
     if not args.no_cpu_baseline:
+        from matrixinversion_amd import workloads
         log("[ragged] timing the CPU baseline ...")
         flat2, offsets2, ns2, _ = workloads.ragged_batch(42, min(batch, 65536), args.n_lo, args.n_hi)
         out["cpu_baseline"] = cpu_baseline_ragged(ns2, flat2, offsets2)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
-    for g in graphs + plain_graphs + packed_graphs:
-        g.close()
-    plan.close()
-    return 0 if same and graph_same and packed_same and bad == 0 else 3
+    return 0 if d["ok"] else 3
+
+
+def load_traffic(name: str) -> dict:
+    """HBM bytes per launch from a committed counter profile (profiles/<name>),
+    kept only when it was measured on the sources of the loaded library
+    (its build_id equals lu_build_id()); otherwise reported as stale."""
+    import matrixinversion_amd as mia
+
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return {"bytes": None, "build_id": None, "stale": None}
+    with open(path) as f:
+        t = json.load(f)
+    stale = t.get("build_id") != mia.build_id()
+    return {"bytes": None if stale else t.get("hbm_bytes_per_launch"), "build_id": t.get("build_id"), "stale": stale}
 
 
 def serve_sweep(rates, duration: float, budget_gb: float = 64.0, pool: int = 200_000) -> list:
@@ -341,6 +396,7 @@ def serve_sweep(rates, duration: float, budget_gb: float = 64.0, pool: int = 200
     pools = (serve.Pool(42, pool), serve.Pool(43, pool, dominant=True))
     out = []
     with serve.Server(int(budget_gb * 2**30)) as srv:
+        serve.warm(srv, pools)  # untimed: first kernel launches load code objects
         for r in rates:
             res = serve.poisson_stream(srv, pools, float(r), duration)
             log(f"[serve] {r:>8.0f} req/s: p50 {res['p50_ms']:.3f} ms, p99 {res['p99_ms']:.3f} ms, "
@@ -588,6 +644,13 @@ def main() -> int:
     ap.add_argument("--rates", default="1000,4000,16000", help="serve: request rates (req/s)")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per rate")
     ap.add_argument("--no-serve", action="store_true", help="headline: skip the short serving run")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline: skip the dominant-input gate and the mixed-n leg (profiling runs)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
+                    help="write the traffic JSON of --traffic-kernel from two rocprofv3 --pmc output folders "
+                         "(FETCH_SIZE, WRITE_SIZE) of a bench run, calibrated by its copy kernel, and exit")
+    ap.add_argument("--traffic-kernel", default="lu_inverse_kernel", help="kernel name prefix for --traffic-from")
+    ap.add_argument("--traffic-out", default=None, help="--traffic-from: output file under profiles/")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -595,6 +658,8 @@ def main() -> int:
               f"build runs on a single MI355X (asked for --gpus {args.gpus}).", file=sys.stderr)
         return 2
 
+    if args.traffic_from:
+        return traffic_from(args)
     if args.workload == "ragged":
         return bench_ragged(args)
     if args.workload == "ragged_classes":
@@ -662,17 +727,16 @@ def main() -> int:
         log("[bench] pivot vectors are not permutations")
         return 3
     log(f"[bench] check: verifyInv correct {correct}/{batch} at 1e-3, max|AX-I|={worst:.2e}")
+    for b in bufs:
+        b.close()
+    pristine.close()
 
     ms_per_step = total_ms / steps
     mats_per_s = batch / (ms_per_step * 1e-3)
     algo_bytes = batch * (8 * n * n + (4 * n if flags != 0 else 0))
     achieved = algo_bytes / (per_launch.mean() * 1e-3) / 1e9
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_lu_inverse_n{n}_{args.pivot}_{batch}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = load_traffic(f"traffic_lu_inverse_n{n}_{args.pivot}_{batch}.json")
 
     out = {
         "metric": METRIC,
@@ -691,16 +755,29 @@ def main() -> int:
                    "n": n, "batch": batch, "pivot": args.pivot, "layout": "row-major contiguous, in place",
                    "global_batch": batch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic["bytes"],
+                     "traffic_build_id": traffic["build_id"], "traffic_stale": traffic["stale"],
+                     "build_id": mia.build_id()},
         "detail": {"kernel_ms_mean": float(per_launch.mean()), "kernel_ms_min": float(per_launch.min()),
                    "kernel_ms_max": float(per_launch.max()), "timed_wall_ms": wall_ms,
                    "algorithmic_bytes_per_launch": algo_bytes, "pct_hbm": 100 * achieved / HBM_PEAK_GBPS,
-                   "verifyInv_correct": correct, "max_abs_AXmI": worst},
+                   "verifyInv_correct": correct, "max_abs_AXmI": worst,
+                   "accuracy_note": "U[-1,1) inputs are not uniformly well-conditioned; the 1e-4 accuracy gate "
+                                    "is checked on diagonally dominant inputs (well_conditioned_1M)"},
     }
-    for b in bufs:
-        b.close()
-    pristine.close()
     piv.close()
+    ok = True
+    if not args.no_extras:
+        # the accuracy target on well-conditioned inputs, at full size: A = U[-1,1) + n*I
+        wc = dominant_gate(n, batch, flags, stream)
+        out["detail"]["well_conditioned_1M"] = wc
+        ok &= wc["gate_passed"]
+        if n == 32 and args.pivot in ("parallel", "serial", "tree"):
+            # the mixed-size target: the ragged path against per-n plain launches
+            log("[bench] mixed-n leg (500k matrices, n~U{2..32}) ...")
+            rg = ragged_measure(500_000, args.pivot, 5, 3)
+            out["detail"]["mixed_n_500k"] = rg
+            ok &= rg["ok"]
     if not args.no_serve:
         # the metric's second half: latency of the mixed serving stream
         top = serve_sweep([16000.0], 2.0)[0]
@@ -714,6 +791,92 @@ def main() -> int:
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+    return 0 if ok else 3
+
+
+def dominant_gate(n: int, batch: int, flags: int, stream) -> dict:
+    """The accuracy target at full size: `batch` matrices A = U[-1,1) + n*I
+    (mt19937(43)), inverted with `flags` on the device; max|A X - I| over the
+    whole batch must be <= 1e-4; the reference's verifyInv count (1e-3) and
+    the count within 1e-4 come from the device check."""
+    import matrixinversion_amd as mia
+    from matrixinversion_amd import workloads
+
+    t0 = time.perf_counter()
+    a = workloads.dominant_matrices(43, batch, n)
+    A = mia.DeviceBuffer.from_array(a)
+    X = mia.DeviceBuffer(a.nbytes)
+    P = mia.DeviceBuffer(batch * n * 4)
+    del a
+    try:
+        X.copy_from(A, stream)
+        mia.lu_batched_inverse(X, n, batch, flags, P, stream)
+        correct3, worst = mia.residual(A, X, n, batch, 1e-3, stream)
+        correct4, _ = mia.residual(A, X, n, batch, 1e-4, stream)
+    finally:
+        for b in (A, X, P):
+            b.close()
+    passed = bool(worst <= 1e-4 and correct4 == batch)
+    log(f"[bench] well-conditioned {batch} x n={n}: max|AX-I|={worst:.2e}, within 1e-4: {correct4}/{batch} "
+        f"({'pass' if passed else 'FAIL'}) [{time.perf_counter() - t0:.1f}s]")
+    return {"inputs": f"U[-1,1) + {n}*I, mt19937(43)", "max_abs_AXmI": worst, "verifyInv_correct_1e-3": correct3,
+            "within_1e-4": correct4, "gate": "max|AX-I| <= 1e-4 for every matrix", "gate_passed": passed}
+
+
+def traffic_from(args) -> int:
+    """HBM bytes per launch of one kernel from two rocprofv3 counter runs of a
+    bench.py command (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
+    scaled by what the counter reads for the copy kernel of the same run,
+    whose bytes are known (it restores every timed input: n*n*batch floats in
+    and out).  Writes profiles/<out> with the loaded library's build id."""
+    import csv
+    import glob
+
+    import matrixinversion_amd as mia
+
+    def per_dispatch(folder, counter):
+        rows = {}
+        for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") != counter:
+                        continue
+                    key = (r["Dispatch_Id"], r["Kernel_Name"], int(r.get("Grid_Size") or 0))
+                    rows[key] = rows.get(key, 0.0) + float(r["Counter_Value"])
+        return rows
+
+    if args.workload == "ragged":
+        from matrixinversion_amd import workloads
+        batch = args.batch or 500_000
+        nn = int((workloads.ragged_sizes(43, batch, args.n_lo, args.n_hi).astype(np.int64) ** 2).sum())
+    else:
+        batch = args.batch or 1_000_000
+        nn = batch * args.n * args.n
+    copy_bytes = (nn // 4) * 16  # the dwordx4 part of one full restore (lubx_copy_f32)
+    out = {}
+    for counter, folder in zip(("FETCH_SIZE", "WRITE_SIZE"), args.traffic_from):
+        rows = per_dispatch(folder, counter)
+        # the full-batch restores are the copy_kernel dispatches with the largest grid
+        grid = max([g for (d, k, g) in rows if "copy_kernel" in k], default=0)
+        copies = [v for (d, k, g), v in rows.items() if "copy_kernel" in k and g == grid]
+        kern = [v for (d, k, g), v in rows.items() if args.traffic_kernel in k]
+        if not copies or not kern:
+            log(f"[traffic] {counter}: no copy_kernel or {args.traffic_kernel} dispatches under {folder}")
+            return 4
+        factor = copy_bytes / (float(np.median(copies)) * 1024)
+        out[counter] = {"kernel_kib": float(np.mean(kern)), "copy_kib": float(np.median(copies)),
+                        "factor": factor, "bytes": float(np.mean(kern)) * 1024 * factor,
+                        "dispatches": len(kern)}
+    total = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    res = {"hbm_bytes_per_launch": total, "fetch_bytes": out["FETCH_SIZE"]["bytes"],
+           "write_bytes": out["WRITE_SIZE"]["bytes"], "fetch_factor": out["FETCH_SIZE"]["factor"],
+           "write_factor": out["WRITE_SIZE"]["factor"], "counters": out, "kernel": args.traffic_kernel,
+           "build_id": mia.build_id(), "copy_bytes_each_way": copy_bytes}
+    path = os.path.join(ROOT, "profiles", args.traffic_out)
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+    log(f"[traffic] {args.traffic_kernel}: {total / 1e9:.4f} GB per launch -> {path}")
+    print(json.dumps(res))
     return 0
 
 

@@ -169,8 +169,11 @@ int lu_batched_inverse_refined(const float* A, float* X, int* piv, int n, int64_
  *
  * Requests are admitted into a device arena of budget_bytes (the HBM
  * admission budget) and stay resident until released.  Queued requests of
- * one pivot rule are coalesced into one ragged batch whenever that rule's
- * stream is idle; requests of different rules run on their own streams.
+ * one pivot rule are coalesced into one ragged batch whenever one of the
+ * server's launch slots (three, each a stream and a ragged plan) is free, so
+ * up to three batches -- of one rule or of several -- run at once, and a
+ * request that arrives while a batch runs starts without waiting for it.
+ * Everything the server uses is allocated by lu_server_create.
  * A result is bit-identical to inverting the same matrices alone.
  * One host thread drives a server; no call blocks except destroy. */
 typedef struct lub_server lub_server;
@@ -196,11 +199,20 @@ int lu_server_progress(lub_server* server, int* launched, int* completed);
  * submit to the progress call that saw completion (-1 before done). */
 int lu_server_query(lub_server* server, uint64_t ticket, int* state, float** result, int** piv, double* latency_ms);
 
+/* Tickets that finished since the last poll, in completion order: up to
+ * `capacity` of them go to tickets[], their number to *count (the rest wait
+ * for the next poll). */
+int lu_server_poll(lub_server* server, uint64_t* tickets, int capacity, int* count);
+
 /* Give a done request's memory back. */
 int lu_server_release(lub_server* server, uint64_t ticket);
 
 /* Matrices queued and running; arena bytes held (admitted, not reclaimed). */
 int lu_server_stats(lub_server* server, int64_t* queued, int64_t* running, int64_t* resident_bytes);
+
+/* Batches launched since create, and the most batches of one pivot rule that
+ * were running at the same time (either pointer may be NULL). */
+int lu_server_counters(lub_server* server, int64_t* batches, int* peak_rule_inflight);
 
 /* Device-side inversion check, the reference's verifyInv (parallel_pivot/
  * verify.hpp:50-103) on the GPU.  For each of `batch` matrices, A (the
@@ -220,8 +232,12 @@ const char* lu_status_string(int status);
 /* Text of the last HIP runtime failure seen by the calling thread ("" if none). */
 const char* lu_last_error(void);
 
-/* Library version, e.g. "0.1.0". */
+/* Library version, e.g. "0.2.0". */
 const char* lu_version(void);
+
+/* Hash of the sources the library was built from (16 hex digits); counter
+ * profiles record it, so that a profile of other code can be told apart. */
+const char* lu_build_id(void);
 
 #ifdef __cplusplus
 }

@@ -40,11 +40,14 @@ ABI = {
     "lu_server_query": (_i, [_vp, ctypes.c_uint64, ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                              ctypes.POINTER(ctypes.c_double)]),
     "lu_server_release": (_i, [_vp, ctypes.c_uint64]),
+    "lu_server_poll": (_i, [_vp, ctypes.POINTER(ctypes.c_uint64), _i, ctypes.POINTER(_i)]),
+    "lu_server_counters": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i)]),
     "lu_server_stats": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "lu_tuning_get": (_i, [ctypes.c_char_p, ctypes.c_char_p]),
     "lu_status_string": (ctypes.c_char_p, [_i]),
     "lu_last_error": (ctypes.c_char_p, []),
     "lu_version": (ctypes.c_char_p, []),
+    "lu_build_id": (ctypes.c_char_p, []),
 }
 
 UTIL = {

@@ -268,3 +268,8 @@ def tuning_set(layout=None, grab=None) -> None:
 
 def version() -> str:
     return lib().lu_version().decode()
+
+
+def build_id() -> str:
+    """Hash of the sources the loaded library was built from."""
+    return lib().lu_build_id().decode()

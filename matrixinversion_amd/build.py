@@ -10,6 +10,7 @@ disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
 from __future__ import annotations
 
 import argparse
+import hashlib
 import os
 import shutil
 import subprocess
@@ -53,6 +54,20 @@ def _run(cmd):
     return time.time() - t0, err
 
 
+def source_id() -> str:
+    """Hash of every source that goes into liblubatch.so (csrc/, include/):
+    compiled into the library as lu_build_id(), and recorded with every
+    counter profile, so a profile taken on other sources shows as stale."""
+    h = hashlib.sha256()
+    for d in (CSRC, INCLUDE):
+        for f in sorted(os.listdir(d)):
+            if f.endswith((".hip", ".hpp", ".h", ".py", ".cpp")):
+                h.update(f.encode() + b"\0")
+                with open(os.path.join(d, f), "rb") as fh:
+                    h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def _stale(cmd, deps_mtime: float) -> bool:
     obj = cmd[-1]
     return not os.path.exists(obj) or os.path.getmtime(obj) < deps_mtime
@@ -79,7 +94,8 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
         units.append([HIPCC, *CFLAGS, f"-DLUB_RAGGED_PIV={piv}", "-c", os.path.join(CSRC, "lu_ragged_inst.hip"),
                       "-o", os.path.join(OBJDIR, f"lu_ragged_{piv}.o")])
     for name in ("lub_api", "lub_util", "lub_verify", "lub_ragged", "lub_serve", "lub_refine", "lub_lapack"):
-        units.append([HIPCC, *CFLAGS, "-c", os.path.join(CSRC, f"{name}.hip"),
+        extra = [f'-DLUB_BUILD_ID="{source_id()}"'] if name == "lub_api" else []
+        units.append([HIPCC, *CFLAGS, *extra, "-c", os.path.join(CSRC, f"{name}.hip"),
                       "-o", os.path.join(OBJDIR, f"{name}.o")])
     todo = units
     if incremental:

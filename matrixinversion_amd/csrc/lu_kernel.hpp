@@ -63,16 +63,20 @@ enum Pivot : int { kPivNone = 0, kPivParallel = 1, kPivSerial = 2, kPivTree = 3,
 // Rows whose length is a multiple of 8 floats get 4 floats of padding so that
 // 8 consecutive rows start in 8 different 16-byte bank groups; for n % 4 == 0
 // consecutive matrices start L banks apart so the lane groups of one
-// half-wave column access hit disjoint banks.  For n % 4 != 0 the tile is
-// stored contiguously (its dwordx4 chunks cross row boundaries).
-template <int N, int L>
+// half-wave column access hit disjoint banks.  For n % 4 != 0 a matrix is
+// stored contiguously (row stride n): in the plain layout (V = 0) the whole
+// tile is contiguous, as in HBM, so the tile's dwordx4 chunks (which cross
+// row and matrix boundaries) stage unchanged; in the gathered layout (V = 1,
+// the ragged kernel, whose matrices arrive one by one) each matrix starts on
+// a 16-byte boundary so that its own chunks stage as aligned dwordx4.
+template <int N, int L, int V = 0>
 struct Geo {
   static_assert(L == 16 || L == 4, "a matrix is owned by a DPP row or a quad");
   static_assert(N >= 1 && N <= (L == 16 ? 32 : 16), "n too large for this lane grouping");
   static constexpr int MPW = kWave / L;          // matrices per wave (tile)
   static constexpr int R = (N + L - 1) / L;      // rows (columns) per lane
   static constexpr int LD = (N % 8 == 0) ? N + 4 : N;
-  static constexpr int MS = (N % 4 == 0) ? ((N * LD + 31) / 32) * 32 + L : N * N;
+  static constexpr int MS = (N % 4 == 0) ? ((N * LD + 31) / 32) * 32 + L : (V == 0 ? N * N : (N * N + 3) / 4 * 4);
   static constexpr int TILE = MPW * MS;
   static constexpr int PERM = TILE;               // int32[MPW][32] after the tile
   static constexpr int LDS_FLOATS = TILE + MPW * 32;
@@ -585,10 +589,10 @@ __device__ __forceinline__ int pivot_failure(const LaneFloats<N, L>& cs, int l) 
 
 // Phases 2-5 of a tile already staged in LDS: pivot positions, P*A in LDS,
 // then columns q, q+L, ... of P*A into the registers of lane q.
-template <int N, int L, int PIV>
+template <int N, int L, int PIV, int V = 0>
 __device__ __forceinline__ void tile_front(ColBlock<N, L>& w, LaneInts<N, L>& prm, float* lds,
                                            int lane) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   const int l = lane % L;
   const int g = lane / L;
   float* const mat = lds + g * G::MS;
@@ -638,10 +642,10 @@ __device__ __forceinline__ void tile_front(ColBlock<N, L>& w, LaneInts<N, L>& pr
 // the matrix's LDS region, which then holds A^-1 (row-major, stride LD).
 // With `failure` non-null, *failure receives pivot_failure() of this lane's
 // matrix.
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ void tile_solve(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* lds, int lane,
                                            int* failure = nullptr) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   const int l = lane % L;
   const int g = lane / L;
   float* const mat = lds + g * G::MS;
@@ -716,9 +720,9 @@ __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L
 // stride LD), perm follows the tile, prm[h] = perm[l + L*h], and *failure
 // (if non-null) the first step whose pivot has no finite reciprocal, or N.
 // A NaN in the pivot column is taken as its largest entry.
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ void tile_gepp(LaneInts<N, L>& prm, float* lds, int lane, int* failure) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   const int l = lane % L;
   const int g = lane / L;
   float* const mat = lds + g * G::MS;

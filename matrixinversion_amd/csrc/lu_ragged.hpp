@@ -57,10 +57,12 @@ struct RaggedWs {
   static constexpr int HEADER = 384;
   static constexpr int CHUNK = 1024;      // matrices per planning block
   static constexpr int BSTRIDE = 33;      // int32 per planning block
-  static int64_t blocks(int64_t cap) { return (cap + CHUNK - 1) / CHUNK; }
-  static int64_t bhist_offset() { return HEADER * 4; }  // bytes
-  static int64_t slots_offset(int64_t cap) { return (bhist_offset() + blocks(cap) * BSTRIDE * 4 + 15) / 16 * 16; }
-  static int64_t bytes(int64_t cap) { return slots_offset(cap) + 2 * 8 * cap; }
+  static constexpr int64_t blocks(int64_t cap) { return (cap + CHUNK - 1) / CHUNK; }
+  static constexpr int64_t bhist_offset() { return HEADER * 4; }  // bytes
+  static constexpr int64_t slots_offset(int64_t cap) {
+    return (bhist_offset() + blocks(cap) * BSTRIDE * 4 + 15) / 16 * 16;
+  }
+  static constexpr int64_t bytes(int64_t cap) { return slots_offset(cap) + 2 * 8 * cap; }
 };
 
 constexpr int kRaggedQuadMaxN = 12;
@@ -73,7 +75,7 @@ __host__ __device__ constexpr int ragged_mpw(int n, int layout) { return (layout
 template <int A_, int B_>
 constexpr int cmax() { return A_ > B_ ? A_ : B_; }
 
-constexpr int kRaggedLdsFloats = cmax<Geo<32, 16>::LDS_FLOATS, Geo<kRaggedQuadMaxN, 4>::LDS_FLOATS>();
+constexpr int kRaggedLdsFloats = cmax<Geo<32, 16, 1>::LDS_FLOATS, Geo<kRaggedQuadMaxN, 4, 1>::LDS_FLOATS>();
 
 // Float4 of a ragged batch: matrices start at any float offset, so global
 // accesses are only 4-byte aligned (gfx950 executes dword-aligned dwordx4).
@@ -99,154 +101,122 @@ __device__ __forceinline__ int64_t slot_lane(uint32_t lo, uint32_t hi, int s) {
   return static_cast<int64_t>((uint64_t(h) << 32) | l);
 }
 
-// Registers of one tile's gather: for n >= 6 MPW matrices of K4 float4
-// chunks per lane (+ one trailing float when n*n % 4 != 0), else ROUNDS
-// floats per lane.
+// Registers of one tile's gather.  Each matrix is loaded by its own lane
+// group (the lanes that will factor it): lane l of group g holds chunks
+// c = l + L*k (k < K; 4 consecutive floats each) of matrix g and, when
+// n*n % 4 != 0, float 4*C4 + l (l < REM) -- so one load instruction reads
+// MPW matrices at once, whatever n is.
 template <int N, int L>
 struct RaggedRegs {
-  static constexpr int NN = N * N;
-  static constexpr bool WHOLE = NN > kWave / 2;
-  static constexpr int C4 = NN / 4, REM = NN % 4, K4 = (C4 + kWave - 1) / kWave;
-  static constexpr int PER = WHOLE ? 1 : kWave / NN;
-  static constexpr int ROUNDS = WHOLE ? 1 : (Geo<N, L>::MPW + PER - 1) / PER;
-  f32x4u v[WHOLE ? Geo<N, L>::MPW : 1][K4];
-  float r[WHOLE ? Geo<N, L>::MPW : ROUNDS];
+  static constexpr int NN = N * N, C4 = NN / 4, REM = NN % 4;
+  static constexpr int K = (C4 + L - 1) / L;
+  f32x4u v[K > 0 ? K : 1];
+  float r;
 };
 
 // Gather loads of up to MPW matrices of one n (slots base .. base+valid-1 of
-// `sl`) into registers; slots past `valid` load the first matrix again, so
-// every load is unconditional and all of them are in flight together.  For
-// n >= 6 the whole wave reads one matrix at a time, 16 bytes per lane (1 KB
-// contiguous per load instruction) plus n*n % 4 trailing floats; for smaller
-// n one dword load instruction covers 64 / (n*n) matrices.
+// `sl`) into registers; groups past `valid` load nothing.  Every lane of the
+// wave must call this (the slot lookup is a cross-lane permute).
 template <int N, int L>
 __device__ __forceinline__ void ragged_load(const float* __restrict__ A, const RaggedSlots& sl, int base, int valid,
                                             RaggedRegs<N, L>& x, int lane) {
-  using G = Geo<N, L>;
   using X = RaggedRegs<N, L>;
-  if constexpr (X::WHOLE) {
+  const int l = lane % L, g = lane / L;
+  const int64_t off = slot_lane(sl.so_lo, sl.so_hi, base + (g < valid ? g : 0));
+  if (g < valid) {
+    const float* src = A + off;
 #pragma unroll
-    for (int g = 0; g < G::MPW; ++g) {
-      const float* src = A + slot_uniform(sl.so_lo, sl.so_hi, base + (g < valid ? g : 0));
-#pragma unroll
-      for (int k = 0; k < X::K4; ++k) {
-        const int c = lane + kWave * k;
-        if (X::C4 % kWave == 0 || c < X::C4)
-          x.v[g][k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u*>(src) + c);
-      }
-      if constexpr (X::REM != 0) x.r[g] = __builtin_nontemporal_load(src + 4 * X::C4 + (lane < X::REM ? lane : 0));
+    for (int k = 0; k < X::K; ++k) {
+      const int c = l + L * k;
+      if (X::C4 % L == 0 || c < X::C4)
+        x.v[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4u*>(src) + c);
     }
-  } else {
-    const int slot = lane / X::NN, e = lane - slot * X::NN;
-#pragma unroll
-    for (int r = 0; r < X::ROUNDS; ++r) {
-      const int g = r * X::PER + slot;
-      const int64_t off = slot_lane(sl.so_lo, sl.so_hi, base + ((slot < X::PER && g < valid) ? g : 0));
-      x.r[r] = __builtin_nontemporal_load(A + off + e);
-    }
+    if constexpr (X::REM != 0) x.r = __builtin_nontemporal_load(src + 4 * X::C4 + (l < X::REM ? l : 0));
   }
 }
 
-// Registers -> the tile's LDS layout; slots past `valid` get the identity.
+// LDS address (floats, within the matrix) of chunk c in the gathered layout:
+// rows of n % 4 == 0 hold whole chunks (row stride LD); other n are stored
+// contiguously from a 16-byte boundary.
+template <int N, int L>
+__device__ __forceinline__ int ragged_chunk_at(int c) {
+  using G = Geo<N, L, 1>;
+  if constexpr (N % 4 == 0) {
+    const int i = (4 * c) / N, j = (4 * c) % N;
+    return i * G::LD + j;
+  } else {
+    return 4 * c;
+  }
+}
+
+// Registers -> the tile's LDS layout (Geo<N, L, 1>), one aligned dwordx4 per
+// chunk; groups past `valid` get the identity.
 template <int N, int L>
 __device__ __forceinline__ void ragged_stage(const RaggedRegs<N, L>& x, int valid, float* lds, int lane) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, 1>;
   using X = RaggedRegs<N, L>;
-  if constexpr (X::WHOLE) {
+  const int l = lane % L, g = lane / L;
+  float* const mat = lds + g * G::MS;
+  if (valid == G::MPW) {  // wave-uniform: full tiles stage the data only
 #pragma unroll
-    for (int g = 0; g < G::MPW; ++g) {
-      float* const mat = lds + g * G::MS;
-      const bool live = g < valid;
-#pragma unroll
-      for (int k = 0; k < X::K4; ++k) {
-        const int c = lane + kWave * k;
-        if (X::C4 % kWave == 0 || c < X::C4) {
-          if constexpr (N % 4 == 0) {
-            // a chunk is 4 consecutive floats of one row; LD and MS keep it 16-byte aligned in LDS
-            const int i = (4 * c) / N, j = (4 * c) % N;
-            f32x4 w;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) w[q] = live ? x.v[g][k][q] : (i == j + q ? 1.0f : 0.0f);
-            *reinterpret_cast<f32x4*>(mat + i * G::LD + j) = w;
-          } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int e = 4 * c + q, i = e / N, j = e % N;
-              mat[i * G::LD + j] = live ? x.v[g][k][q] : (i == j ? 1.0f : 0.0f);
-            }
-          }
-        }
-      }
-      if constexpr (X::REM != 0) {
-        if (lane < X::REM) {
-          const int e = 4 * X::C4 + lane, i = e / N, j = e % N;
-          mat[i * G::LD + j] = live ? x.r[g] : (i == j ? 1.0f : 0.0f);
-        }
+    for (int k = 0; k < X::K; ++k) {
+      const int c = l + L * k;
+      if (X::C4 % L == 0 || c < X::C4) {
+        const f32x4u v = x.v[k];
+        *reinterpret_cast<f32x4*>(mat + ragged_chunk_at<N, L>(c)) = f32x4{v[0], v[1], v[2], v[3]};
       }
     }
+    if constexpr (X::REM != 0)
+      if (l < X::REM) mat[4 * X::C4 + l] = x.r;
   } else {
-    const int slot = lane / X::NN, e = lane - slot * X::NN;
-    const int i = e / N, j = e - i * N;
+    // A partial tile: groups past `valid` get the identity, as zeros and then
+    // the diagonal (the same wave's LDS writes land in order).  Nothing here
+    // is a function of the lane alone, so nothing is hoisted out of the tile
+    // loop into registers that would stay live through the factorisation.
+    const bool live = g < valid;
 #pragma unroll
-    for (int r = 0; r < X::ROUNDS; ++r) {
-      const int g = r * X::PER + slot;
-      if (slot < X::PER && g < G::MPW)
-        lds[g * G::MS + i * G::LD + j] = (g < valid) ? x.r[r] : (i == j ? 1.0f : 0.0f);
+    for (int k = 0; k < X::K; ++k) {
+      const int c = l + L * k;
+      if (X::C4 % L == 0 || c < X::C4) {
+        const f32x4u v = x.v[k];
+        const f32x4 w = live ? f32x4{v[0], v[1], v[2], v[3]} : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        *reinterpret_cast<f32x4*>(mat + ragged_chunk_at<N, L>(c)) = w;
+      }
+    }
+    if constexpr (X::REM != 0)
+      if (l < X::REM) mat[4 * X::C4 + l] = live ? x.r : 0.0f;
+    if (!live) {
+#pragma unroll
+      for (int h = 0; h < G::R; ++h) {
+        const int i = l + L * h;
+        if (i < N) mat[i * G::LD + i] = 1.0f;
+      }
     }
   }
 }
 
-// The mirror of ragged_load + ragged_stage for the `valid` leading matrices of the tile.
+// The mirror of ragged_load + ragged_stage for the `valid` leading matrices
+// of the tile; every lane of the wave must call it.
 template <int N, int L>
 __device__ __forceinline__ void ragged_scatter_store(float* __restrict__ A, const RaggedSlots& sl, int base,
                                                      int valid, const float* lds, int lane) {
-  using G = Geo<N, L>;
-  constexpr int NN = N * N;
-  if constexpr (NN > kWave / 2) {
-    constexpr int C4 = NN / 4, REM = NN % 4, K4 = (C4 + kWave - 1) / kWave;
+  using G = Geo<N, L, 1>;
+  using X = RaggedRegs<N, L>;
+  const int l = lane % L, g = lane / L;
+  const int64_t off = slot_lane(sl.so_lo, sl.so_hi, base + (g < valid ? g : 0));
+  if (g < valid) {
+    const float* const mat = lds + g * G::MS;
+    float* dst = A + off;
 #pragma unroll
-    for (int g = 0; g < G::MPW; ++g) {
-      if (g < valid) {
-        const float* const mat = lds + g * G::MS;
-        float* dst = A + slot_uniform(sl.so_lo, sl.so_hi, base + g);
-#pragma unroll
-        for (int k = 0; k < K4; ++k) {
-          const int c = lane + kWave * k;
-          if (C4 % kWave == 0 || c < C4) {
-            f32x4u w;
-            if constexpr (N % 4 == 0) {
-              const int i = (4 * c) / N, j = (4 * c) % N;
-              const f32x4 t = *reinterpret_cast<const f32x4*>(mat + i * G::LD + j);
-              w = f32x4u{t[0], t[1], t[2], t[3]};
-            } else {
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                const int e = 4 * c + q;
-                w[q] = mat[(e / N) * G::LD + e % N];
-              }
-            }
-            __builtin_nontemporal_store(w, reinterpret_cast<f32x4u*>(dst) + c);
-          }
-        }
-        if constexpr (REM != 0) {
-          if (lane < REM) {
-            const int e = 4 * C4 + lane;
-            __builtin_nontemporal_store(mat[(e / N) * G::LD + e % N], dst + e);
-          }
-        }
+    for (int k = 0; k < X::K; ++k) {
+      const int c = l + L * k;
+      if (X::C4 % L == 0 || c < X::C4) {
+        const f32x4 t = *reinterpret_cast<const f32x4*>(mat + ragged_chunk_at<N, L>(c));
+        __builtin_nontemporal_store(f32x4u{t[0], t[1], t[2], t[3]}, reinterpret_cast<f32x4u*>(dst) + c);
       }
     }
-  } else {
-    constexpr int PER = kWave / NN;
-    constexpr int ROUNDS = (G::MPW + PER - 1) / PER;
-    const int slot = lane / NN, e = lane - slot * NN;
-    const int i = e / N, j = e - i * N;
-#pragma unroll
-    for (int r = 0; r < ROUNDS; ++r) {
-      const int g = r * PER + slot;
-      const int64_t off = slot_lane(sl.so_lo, sl.so_hi, base + ((slot < PER && g < valid) ? g : 0));
-      if (slot < PER && g < valid) __builtin_nontemporal_store(lds[g * G::MS + i * G::LD + j], A + off + e);
-    }
+    if constexpr (X::REM != 0)
+      if (l < X::REM) __builtin_nontemporal_store(mat[4 * X::C4 + l], dst + 4 * X::C4 + l);
   }
 }
 
@@ -258,7 +228,7 @@ __device__ __forceinline__ void ragged_scatter_store(float* __restrict__ A, cons
 template <int N, int L, int PIV>
 __device__ __forceinline__ void ragged_tile(float* __restrict__ A, int* __restrict__ piv, const RaggedSlots& sl,
                                             int count, float* lds, int lane) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, 1>;
   static_assert(G::LDS_FLOATS <= kRaggedLdsFloats, "tile exceeds the ragged kernel's LDS array");
   const int l = lane % L;
   const int g = lane / L;
@@ -270,11 +240,11 @@ __device__ __forceinline__ void ragged_tile(float* __restrict__ A, int* __restri
     wave_sync();
     LaneInts<N, L> prm;
     if constexpr (PIV == kPivGepp) {
-      tile_gepp<N, L>(prm, lds, lane, nullptr);
+      tile_gepp<N, L, 1>(prm, lds, lane, nullptr);
     } else {
       ColBlock<N, L> w;
-      tile_front<N, L, PIV>(w, prm, lds, lane);
-      tile_solve<N, L>(w, prm, lds, lane);
+      tile_front<N, L, PIV, 1>(w, prm, lds, lane);
+      tile_solve<N, L, 1>(w, prm, lds, lane);
     }
     auto finish = [&](int b0, int nv) {
       ragged_scatter_store<N, L>(A, sl, b0, nv, lds, lane);

@@ -243,7 +243,13 @@ const char* lu_status_string(int status) {
   }
 }
 
-const char* lu_version(void) { return "0.1.0"; }
+const char* lu_version(void) { return "0.2.0"; }
+
+// Hash of the sources this library was built from (set by the build).
+#ifndef LUB_BUILD_ID
+#define LUB_BUILD_ID "unknown"
+#endif
+const char* lu_build_id(void) { return LUB_BUILD_ID; }
 
 // Text of the last HIP failure seen by this thread (empty if none).
 const char* lu_last_error(void) { return g_last_error; }

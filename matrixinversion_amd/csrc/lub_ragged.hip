@@ -224,15 +224,64 @@ int ragged_per(const lub::RaggedTuning& rt, int n) {
 // (ceil(batch / smallest matrices-per-descriptor) + 32 bounds it), so each
 // workgroup does one descriptor; LUB_RAGGED_WAVES_PER_CU=<k> launches k
 // persistent waves per compute unit instead.
-int64_t ragged_grid(const lub_ragged_plan* p, const lub::RaggedTuning& rt, int64_t batch) {
+int64_t ragged_grid(int cus, const lub::RaggedTuning& rt, int64_t batch) {
   static const int per_cu = [] {
     const char* e = std::getenv("LUB_RAGGED_WAVES_PER_CU");
     return e != nullptr ? std::atoi(e) : 0;
   }();
-  if (per_cu > 0) return int64_t(p->cus) * per_cu;
+  if (per_cu > 0 && cus > 0) return int64_t(cus) * per_cu;
   int min_per = lub::kRaggedMaxPer;
   for (int n = 1; n <= 32; ++n) min_per = std::min(min_per, ragged_per(rt, n));
   return std::min<int64_t>((batch + min_per - 1) / min_per + 32, int64_t(1) << 25);  // grid * 64 within 32 bits
+}
+
+int check_ragged_args(float* A, const int64_t* offsets, const int* ns, int* piv, const int64_t* piv_offsets,
+                      int64_t batch, int flags) {
+  if (batch < 0 || batch > kMaxRaggedBatch) return LUB_EINVAL_BATCH;
+  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || (flags & LUB_PIVOT_MASK) > LUB_PIVOT_GEPP ||
+      (flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_MASK)
+    return LUB_EFLAGS;
+  if (batch == 0) return LUB_OK;
+  if (A == nullptr || offsets == nullptr || ns == nullptr) return LUB_ENULL;
+  if (piv != nullptr && piv_offsets == nullptr) return LUB_ENULL;
+  return -1;  // proceed
+}
+
+// The four launches of one ragged batch on `stream`, with the planning
+// workspace at `ws` (lub::RaggedWs layout for `capacity` matrices, batch <=
+// capacity, arguments already checked).  No allocation, no synchronisation.
+int enqueue_ragged(char* ws, int64_t capacity, int cus, float* A, const int64_t* offsets, const int* ns, int* piv,
+                   const int64_t* piv_offsets, int64_t batch, int flags, hipStream_t s) {
+  const int pv = flags & LUB_PIVOT_MASK;
+  const void* fn = lub::kernel_table().ragged[pv];
+  if (fn == nullptr) return LUB_ENODEVICE;
+
+  lub::Tuning tuning = lub::current_tuning();
+  lub::RaggedTuning rt;
+  for (int n = 0; n <= 32; ++n) {
+    int layout = tuning.layout[n];
+    if ((flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_ROW16) layout = 0;
+    if ((flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_QUAD4) layout = 1;
+    rt.layout[n] = static_cast<int8_t>(layout);
+    rt.grab[n] = tuning.grab[n];
+  }
+  int* hdr = reinterpret_cast<int*>(ws);
+  int* bhist = reinterpret_cast<int*>(ws + lub::RaggedWs::bhist_offset());
+  int64_t* soffs = reinterpret_cast<int64_t*>(ws + lub::RaggedWs::slots_offset(capacity));
+  int64_t* spoffs_all = soffs + capacity;
+  const unsigned blocks = plan_blocks(batch);
+  const int64_t nblocks = blocks;
+  lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, bhist, nblocks);
+  lub::ragged_scan<<<32, lub::kPlanThreads, 0, s>>>(hdr, bhist, nblocks);
+  lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, hdr,
+                                                            bhist, nblocks, rt, soffs, spoffs_all);
+  int64_t* spoffs = piv ? spoffs_all : nullptr;
+  const int* hdr_arg = hdr;
+  void* args[] = {&A, &soffs, &piv, &spoffs, &hdr_arg};
+  const int64_t grid = ragged_grid(cus, rt, batch);
+  hipError_t e = hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(64), args, 0, s);
+  if (e != hipSuccess) return LUB_EHIP;
+  return hipGetLastError() == hipSuccess ? LUB_OK : LUB_EHIP;
 }
 
 struct ThreadPlan {
@@ -293,44 +342,11 @@ int lu_ragged_plan_destroy(lub_ragged_plan* plan) {
 int lu_batched_inverse_ragged_plan(lub_ragged_plan* plan, float* A, const int64_t* offsets, const int* ns, int* piv,
                                    const int64_t* piv_offsets, int64_t batch, int flags, void* stream) {
   if (plan == nullptr) return LUB_ENULL;
-  if (batch < 0 || batch > kMaxRaggedBatch) return LUB_EINVAL_BATCH;
-  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || (flags & LUB_PIVOT_MASK) > LUB_PIVOT_GEPP ||
-      (flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_MASK)
-    return LUB_EFLAGS;
-  if (batch == 0) return LUB_OK;
-  if (A == nullptr || offsets == nullptr || ns == nullptr) return LUB_ENULL;
-  if (piv != nullptr && piv_offsets == nullptr) return LUB_ENULL;
+  const int rc = check_ragged_args(A, offsets, ns, piv, piv_offsets, batch, flags);
+  if (rc >= 0) return rc;
   if (batch > plan->capacity) return LUB_EINVAL_ARG;
-  const int pv = flags & LUB_PIVOT_MASK;
-  const void* fn = lub::kernel_table().ragged[pv];
-  if (fn == nullptr) return LUB_ENODEVICE;
-
-  lub::Tuning tuning = lub::current_tuning();
-  lub::RaggedTuning rt;
-  for (int n = 0; n <= 32; ++n) {
-    int layout = tuning.layout[n];
-    if ((flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_ROW16) layout = 0;
-    if ((flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_QUAD4) layout = 1;
-    rt.layout[n] = static_cast<int8_t>(layout);
-    rt.grab[n] = tuning.grab[n];
-  }
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  int* hdr = plan->header();
-  const unsigned blocks = plan_blocks(batch);
-  int* bhist = plan->bhist();
-  const int64_t nblocks = blocks;
-  lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, bhist, nblocks);
-  lub::ragged_scan<<<32, lub::kPlanThreads, 0, s>>>(hdr, bhist, nblocks);
-  lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, hdr,
-                                                            bhist, nblocks, rt, plan->soffs(), plan->spoffs());
-  int64_t* soffs = plan->soffs();
-  int64_t* spoffs = piv ? plan->spoffs() : nullptr;
-  const int* hdr_arg = hdr;
-  void* args[] = {&A, &soffs, &piv, &spoffs, &hdr_arg};
-  const int64_t grid = ragged_grid(plan, rt, batch);
-  hipError_t e = hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(64), args, 0, s);
-  if (e != hipSuccess) return LUB_EHIP;
-  return hipGetLastError() == hipSuccess ? LUB_OK : LUB_EHIP;
+  return enqueue_ragged(plan->ws, plan->capacity, plan->cus, A, offsets, ns, piv, piv_offsets, batch, flags,
+                        static_cast<hipStream_t>(stream));
 }
 
 int lu_batched_inverse_ragged(float* A, const int64_t* offsets, const int* ns, int* piv, const int64_t* piv_offsets,

@@ -5,6 +5,11 @@
 // one batch per process (parallel_pivot/run.py:198-234).  Here requests of
 // any size mix arrive one at a time and are inverted in HBM-resident batches:
 //
+//   create    everything the server will use is made here, so that no
+//             request ever waits for an allocation: the arena (the HBM
+//             budget), a copy stream, kSlots launch slots (a compute stream,
+//             a ragged plan for max_batch matrices and two events each), and
+//             per pivot rule a device slot ring plus its pinned staging ring.
 //   submit    a request (count matrices, packed row-major, sizes ns[]) is
 //             admitted if its block fits the HBM budget: the block (its
 //             matrices, then its pivot vectors) is carved from a ring arena in
@@ -13,12 +18,14 @@
 //             written to the pinned staging ring of its pivot rule.
 //             Otherwise LUB_EBUSY: the caller keeps the request and retries
 //             after progress/release.
-//   progress  retires batches whose completion event has fired, then, for
-//             each pivot rule whose stream is idle, launches every queued
-//             request of that rule (contiguous in its slot ring) as one ragged
-//             batch: the batch's slot entries go up in one copy per array, the
-//             compute stream waits for the copies, runs the ragged plan and
-//             records the batch's completion event.
+//   progress  retires batches whose completion event has fired, then, while a
+//             launch slot is free, launches the queued requests of a pivot
+//             rule (the front run of its ring) as one ragged batch on that
+//             slot: the batch's slot entries go up in one copy per array, the
+//             slot's stream waits for the copies, runs the ragged plan and
+//             records the batch's completion event.  Batches on different
+//             slots -- of one rule or of several -- run concurrently.
+//   poll      the tickets finished since the last poll, in completion order.
 //   query     state, device result pointers and end-to-end latency (host clock,
 //             submit to the progress call that saw the batch complete).
 //   release   the caller is done with the results; arena space is reclaimed
@@ -41,6 +48,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int kRules = 5;                 // LUB_PIVOT_NONE .. LUB_PIVOT_GEPP
+constexpr int kSlots = 3;                 // batches in flight at once (+ the copy stream = 4 streams)
 constexpr int64_t kAlign = 256;           // request blocks start on 256-byte boundaries
 
 int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -61,15 +69,25 @@ struct Request {
   Clock::time_point t_submit, t_done;
 };
 
-struct Batch {
-  hipEvent_t done = nullptr;
-  int64_t slot_end = 0;           // monotonic ring position after the batch
+// A launch slot: one batch at a time on its own stream and plan.
+struct Slot {
+  hipStream_t stream = nullptr;
+  lub_ragged_plan* plan = nullptr;
+  hipEvent_t copied = nullptr;   // the batch's slot entries are on the device
+  hipEvent_t done = nullptr;     // the batch is inverted
+  bool busy = false;
+  int rule = -1;
+  int64_t ring_end = 0;          // monotonic ring position after the batch
   std::vector<uint64_t> tickets;
 };
 
+// A launched batch's run of its rule's ring, in launch (= ring) order.
+struct RingRun {
+  int64_t end = 0;
+  bool done = false;
+};
+
 struct Rule {
-  hipStream_t stream = nullptr;
-  lub_ragged_plan* plan = nullptr;
   int64_t* d_offs = nullptr;      // device slot ring
   int* d_ns = nullptr;
   int64_t* d_poffs = nullptr;
@@ -78,7 +96,8 @@ struct Rule {
   int64_t* h_poffs = nullptr;
   int64_t head = 0, tail = 0;     // monotonic slot positions
   std::deque<uint64_t> queued;    // tickets in submission order
-  std::deque<Batch> inflight;
+  std::deque<RingRun> runs;       // launched batches not yet reclaimed
+  int inflight = 0;               // batches of this rule running now
 };
 
 }  // namespace
@@ -90,13 +109,15 @@ struct lub_server {
   int64_t head = 0, tail = 0;     // monotonic arena positions
   int64_t max_batch = 0;
   int64_t ring = 0;               // slots per rule ring
-  int max_inflight = 1;
   hipStream_t copy = nullptr;
+  Slot slots[kSlots];
   Rule rules[kRules];
   std::deque<Request> reqs;       // reqs[i].ticket == first_ticket + i
   uint64_t first_ticket = 1;
   uint64_t next_ticket = 1;
-  std::vector<hipEvent_t> event_pool;
+  std::vector<uint64_t> finished; // done since the last poll
+  int64_t batches = 0;            // launched in all
+  int peak_rule_inflight = 0;     // most batches of one rule running at once
 };
 
 namespace {
@@ -106,22 +127,27 @@ Request* find(lub_server* s, uint64_t t) {
   return &s->reqs[t - s->first_ticket];
 }
 
-hipEvent_t take_event(lub_server* s) {
-  if (!s->event_pool.empty()) {
-    hipEvent_t e = s->event_pool.back();
-    s->event_pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
-  return e;
+int slot_init(lub_server* s, Slot& sl) {
+  if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) return LUB_EHIP;
+  int rc = lu_ragged_plan_create(&sl.plan, s->max_batch);
+  if (rc != LUB_OK) return rc;
+  if (hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+    return LUB_EHIP;
+  sl.tickets.reserve(1024);
+  return LUB_OK;
+}
+
+void slot_free(Slot& sl) {
+  if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+  if (sl.plan) lu_ragged_plan_destroy(sl.plan);
+  if (sl.copied) (void)hipEventDestroy(sl.copied);
+  if (sl.done) (void)hipEventDestroy(sl.done);
+  if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  sl = Slot{};
 }
 
 int rule_init(lub_server* s, Rule& r) {
-  if (r.stream) return LUB_OK;
-  if (hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking) != hipSuccess) return LUB_EHIP;
-  int rc = lu_ragged_plan_create(&r.plan, s->max_batch);
-  if (rc != LUB_OK) return rc;
   const size_t n = static_cast<size_t>(s->ring);
   if (hipMalloc(&r.d_offs, n * 8) != hipSuccess || hipMalloc(&r.d_ns, n * 4) != hipSuccess ||
       hipMalloc(&r.d_poffs, n * 8) != hipSuccess)
@@ -134,17 +160,12 @@ int rule_init(lub_server* s, Rule& r) {
 }
 
 void rule_free(Rule& r) {
-  if (r.stream) (void)hipStreamSynchronize(r.stream);
-  for (Batch& b : r.inflight)
-    if (b.done) (void)hipEventDestroy(b.done);
-  if (r.plan) lu_ragged_plan_destroy(r.plan);
   if (r.d_offs) (void)hipFree(r.d_offs);
   if (r.d_ns) (void)hipFree(r.d_ns);
   if (r.d_poffs) (void)hipFree(r.d_poffs);
   if (r.h_offs) (void)hipHostFree(r.h_offs);
   if (r.h_ns) (void)hipHostFree(r.h_ns);
   if (r.h_poffs) (void)hipHostFree(r.h_poffs);
-  if (r.stream) (void)hipStreamDestroy(r.stream);
   r = Rule{};
 }
 
@@ -174,47 +195,51 @@ void reclaim(lub_server* s) {
 
 int retire(lub_server* s, int* completed) {
   const Clock::time_point now = Clock::now();
-  for (Rule& r : s->rules) {
-    while (!r.inflight.empty()) {
-      Batch& b = r.inflight.front();
-      const hipError_t q = hipEventQuery(b.done);
-      if (q == hipErrorNotReady) break;
-      if (q != hipSuccess) return LUB_EHIP;
-      for (uint64_t t : b.tickets) {
-        Request* rq = find(s, t);
-        rq->state = kDone;
-        rq->t_done = now;
-      }
-      if (completed) *completed += static_cast<int>(b.tickets.size());
-      r.tail = b.slot_end;
-      s->event_pool.push_back(b.done);
-      r.inflight.pop_front();
+  for (Slot& sl : s->slots) {
+    if (!sl.busy) continue;
+    const hipError_t q = hipEventQuery(sl.done);
+    if (q == hipErrorNotReady) continue;
+    if (q != hipSuccess) return LUB_EHIP;
+    for (uint64_t t : sl.tickets) {
+      Request* rq = find(s, t);
+      rq->state = kDone;
+      rq->t_done = now;
+      s->finished.push_back(t);
     }
+    if (completed) *completed += static_cast<int>(sl.tickets.size());
+    Rule& r = s->rules[sl.rule];
+    for (RingRun& run : r.runs)
+      if (run.end == sl.ring_end) run.done = true;
+    // ring space comes back in ring order
+    while (!r.runs.empty() && r.runs.front().done) {
+      r.tail = r.runs.front().end;
+      r.runs.pop_front();
+    }
+    --r.inflight;
+    sl.busy = false;
+    sl.tickets.clear();
   }
   return LUB_OK;
 }
 
-int launch(lub_server* s, int rule, int* launched) {
+// Launch the front run of `rule`'s queue on `sl`.  Tickets leave the queue
+// only once every enqueue has succeeded; on an error they stay queued (and
+// the slot stays free), so a later progress call retries them.
+int launch(lub_server* s, int rule, Slot& sl, int* launched) {
   Rule& r = s->rules[rule];
-  if (r.queued.empty() || static_cast<int>(r.inflight.size()) >= s->max_inflight) return LUB_OK;
-  Batch b;
   const Request* first = find(s, r.queued.front());
   const int64_t begin = first->slot;
-  int64_t end = begin;
   const int flags = first->flags;
-  while (!r.queued.empty()) {
-    const Request* rq = find(s, r.queued.front());
+  int64_t end = begin;
+  size_t take = 0;
+  for (uint64_t t : r.queued) {
+    const Request* rq = find(s, t);
     // a batch is one contiguous run of ring slots with one set of flags
     if (rq->slot != end || rq->flags != flags || end - begin + rq->count > s->max_batch) break;
     if ((begin % s->ring) + (end - begin) + rq->count > s->ring) break;
     end += rq->count;
-    b.tickets.push_back(rq->ticket);
-    r.queued.pop_front();
+    ++take;
   }
-  if (b.tickets.empty()) return LUB_OK;
-  hipEvent_t copied = take_event(s);
-  b.done = take_event(s);
-  if (!copied || !b.done) return LUB_EHIP;
   // the batch's slot entries in one upload per array (staged by submit)
   const int64_t at = begin % s->ring;
   const size_t cnt = static_cast<size_t>(end - begin);
@@ -222,20 +247,34 @@ int launch(lub_server* s, int rule, int* launched) {
       hipMemcpyAsync(r.d_ns + at, r.h_ns + at, cnt * 4, hipMemcpyHostToDevice, s->copy) != hipSuccess ||
       hipMemcpyAsync(r.d_poffs + at, r.h_poffs + at, cnt * 8, hipMemcpyHostToDevice, s->copy) != hipSuccess)
     return LUB_EHIP;
-  if (hipEventRecord(copied, s->copy) != hipSuccess) return LUB_EHIP;
-  if (hipStreamWaitEvent(r.stream, copied, 0) != hipSuccess) return LUB_EHIP;
+  if (hipEventRecord(sl.copied, s->copy) != hipSuccess) return LUB_EHIP;
+  if (hipStreamWaitEvent(sl.stream, sl.copied, 0) != hipSuccess) return LUB_EHIP;
   float* A = reinterpret_cast<float*>(s->arena);
   int* piv = rule == LUB_PIVOT_NONE ? nullptr : reinterpret_cast<int*>(s->arena);
-  int rc = lu_batched_inverse_ragged_plan(r.plan, A, r.d_offs + at, r.d_ns + at, piv, piv ? r.d_poffs + at : nullptr,
-                                          end - begin, flags, r.stream);
-  if (rc != LUB_OK) return rc;
-  if (hipEventRecord(b.done, r.stream) != hipSuccess) return LUB_EHIP;
-  s->event_pool.push_back(copied);  // reusable once recorded work is ordered behind it
-  for (uint64_t t : b.tickets) find(s, t)->state = kRunning;
-  b.slot_end = end;
-  r.inflight.push_back(std::move(b));
+  int rc = lu_batched_inverse_ragged_plan(sl.plan, A, r.d_offs + at, r.d_ns + at, piv,
+                                          piv ? r.d_poffs + at : nullptr, end - begin, flags, sl.stream);
+  if (rc != LUB_OK) return rc;  // the inversion kernel did not start: the batch can run again later
+  // From here the batch is committed (its matrices are being inverted in
+  // place, so it must never be launched a second time).
+  const bool recorded = hipEventRecord(sl.done, sl.stream) == hipSuccess;
+  if (!recorded) (void)hipStreamSynchronize(sl.stream);  // then `done` (never recorded) reads as complete
+
+  sl.tickets.clear();
+  for (size_t i = 0; i < take; ++i) {
+    const uint64_t t = r.queued.front();
+    r.queued.pop_front();
+    find(s, t)->state = kRunning;
+    sl.tickets.push_back(t);
+  }
+  sl.busy = true;
+  sl.rule = rule;
+  sl.ring_end = end;
+  r.runs.push_back(RingRun{end, false});
+  ++r.inflight;
+  ++s->batches;
+  s->peak_rule_inflight = std::max(s->peak_rule_inflight, r.inflight);
   if (launched) ++*launched;
-  return LUB_OK;
+  return recorded ? LUB_OK : LUB_EHIP;
 }
 
 }  // namespace
@@ -250,10 +289,15 @@ int lu_server_create(lub_server** out, int64_t budget_bytes, int64_t max_batch, 
   s->arena_bytes = budget_bytes / kAlign * kAlign;
   s->max_batch = max_batch;
   s->ring = ring_slots;
+  int rc = LUB_OK;
   if (hipGetDevice(&s->device) != hipSuccess || hipMalloc(&s->arena, s->arena_bytes) != hipSuccess ||
-      hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess)
+    rc = LUB_EHIP;
+  for (int i = 0; rc == LUB_OK && i < kSlots; ++i) rc = slot_init(s, s->slots[i]);
+  for (int i = 0; rc == LUB_OK && i < kRules; ++i) rc = rule_init(s, s->rules[i]);
+  if (rc != LUB_OK) {
     lu_server_destroy(s);
-    return LUB_EHIP;
+    return rc;
   }
   *out = s;
   return LUB_OK;
@@ -262,8 +306,8 @@ int lu_server_create(lub_server** out, int64_t budget_bytes, int64_t max_batch, 
 int lu_server_destroy(lub_server* s) {
   if (s == nullptr) return LUB_ENULL;
   if (s->copy) (void)hipStreamSynchronize(s->copy);
+  for (Slot& sl : s->slots) slot_free(sl);
   for (Rule& r : s->rules) rule_free(r);
-  for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
   if (s->copy) (void)hipStreamDestroy(s->copy);
   if (s->arena) (void)hipFree(s->arena);
   delete s;
@@ -284,8 +328,6 @@ int lu_server_submit(lub_server* s, const float* src, const int* ns, int count, 
     pivs += ns[i];
   }
   Rule& r = s->rules[rule];
-  int rc = rule_init(s, r);
-  if (rc != LUB_OK) return rc;
   // slot ring: contiguous run of `count` slots, skipping the ring's end
   if (r.head == r.tail) r.head = r.tail = 0;  // empty ring: restart at its beginning
   int64_t slot = r.head;
@@ -335,10 +377,29 @@ int lu_server_progress(lub_server* s, int* launched, int* completed) {
   if (completed) *completed = 0;
   int rc = retire(s, completed);
   if (rc != LUB_OK) return rc;
-  for (int rule = 0; rule < kRules; ++rule) {
-    rc = launch(s, rule, launched);
+  // Free slots go to the rules with queued work, the rule with the fewest
+  // batches running first (ties: lower rule number).
+  for (Slot& sl : s->slots) {
+    if (sl.busy) continue;
+    int pick = -1;
+    for (int rule = 0; rule < kRules; ++rule) {
+      if (s->rules[rule].queued.empty()) continue;
+      if (pick < 0 || s->rules[rule].inflight < s->rules[pick].inflight) pick = rule;
+    }
+    if (pick < 0) break;
+    rc = launch(s, pick, sl, launched);
     if (rc != LUB_OK) return rc;
   }
+  return LUB_OK;
+}
+
+int lu_server_poll(lub_server* s, uint64_t* tickets, int capacity, int* count) {
+  if (s == nullptr || count == nullptr || (tickets == nullptr && capacity > 0)) return LUB_ENULL;
+  if (capacity < 0) return LUB_EINVAL_ARG;
+  const int take = static_cast<int>(std::min<size_t>(s->finished.size(), static_cast<size_t>(capacity)));
+  std::copy(s->finished.begin(), s->finished.begin() + take, tickets);
+  s->finished.erase(s->finished.begin(), s->finished.begin() + take);
+  *count = take;
   return LUB_OK;
 }
 
@@ -378,6 +439,13 @@ int lu_server_stats(lub_server* s, int64_t* queued, int64_t* running, int64_t* r
   if (queued) *queued = q;
   if (running) *running = run;
   if (resident_bytes) *resident_bytes = s->head - s->tail;
+  return LUB_OK;
+}
+
+int lu_server_counters(lub_server* s, int64_t* batches, int* peak_rule_inflight) {
+  if (s == nullptr) return LUB_ENULL;
+  if (batches) *batches = s->batches;
+  if (peak_rule_inflight) *peak_rule_inflight = s->peak_rule_inflight;
   return LUB_OK;
 }
 

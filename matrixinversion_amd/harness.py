@@ -17,6 +17,15 @@ std_dev, incorrect_inversions (now populated) -- plus throughput and HBM
 fractions.  Output layout: <out>/runtime_results/<100k|500k|1M|...>/
 benchmark_results_<label>.json, keys "m{n}_n{batch}_t{threads}".
 
+Beyond the reference's keys each record carries max_abs_AXmI (device),
+max_rel_err_vs_fp64 (max |X - A^-1| / max |A^-1| against numpy's FP64 inverse,
+over the first CHECK_SAMPLE matrices), pivot_mismatches (matrices of that
+sample whose pivot vector differs from the reference driver's own host
+check, pivotedA: first maximum |a| of the un-updated column,
+parallel_pivot/luBatchedInplace.cu:148-149 -> verify.hpp:106-155; null for the
+tree and partial-pivoting rules, which pivotedA does not describe) and
+cpu_cores (host cores available to the process).
+
 Inputs: "random" (default) fills every matrix from one mt19937 stream (seeded
 per n) with a = 2*((x>>8)*2^-24) - 1; "dominant" adds n*I; "template:<file>"
 reproduces the reference's input construction -- the first n*n numbers of a
@@ -41,6 +50,56 @@ from .workloads import make_inputs
 
 HBM_PEAK_GBPS = 8000.0
 THREADS_PER_WORKGROUP = 64  # one wave owns a tile of 4 matrices
+CHECK_SAMPLE = 1024         # matrices checked on the host per configuration
+
+
+def cpu_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def pivoted_a(a: np.ndarray) -> np.ndarray:
+    """The reference driver's host pivot check (pivotedA, parallel_pivot/
+    verify.hpp:106-155) over a batch: at step k the row at positions k..n-1
+    with the largest |a[.][k]| of the row-permuted input (no elimination) is
+    swapped to position k; strict '>' so the first maximum wins, a NaN at
+    position k is never beaten and a NaN elsewhere never wins.  Returns
+    perm (batch, n): perm[p] = original row at position p."""
+    b, n, _ = a.shape
+    perm = np.tile(np.arange(n, dtype=np.int64), (b, 1))
+    mag = np.abs(a.astype(np.float32))
+    idx = np.arange(b)
+    for k in range(n):
+        vals = np.take_along_axis(mag[:, :, k], perm[:, k:], axis=1)
+        keep = np.isnan(vals[:, 0])
+        best = np.argmax(np.where(np.isnan(vals), -np.inf, vals), axis=1) + k
+        best = np.where(keep, k, best)
+        pk = perm[idx, k].copy()
+        perm[idx, k] = perm[idx, best]
+        perm[idx, best] = pk
+    return perm.astype(np.int32)
+
+
+def host_checks(a: np.ndarray, x: np.ndarray, piv: np.ndarray, flags: int):
+    """(max_rel_err_vs_fp64, pivot_mismatches) over a sample: a, x (count, n, n),
+    piv (count, n)."""
+    a64 = a.astype(np.float64)
+    try:
+        inv = np.linalg.inv(a64)
+    except np.linalg.LinAlgError:
+        inv = np.stack([np.linalg.pinv(m) for m in a64])
+    num = np.abs(x.astype(np.float64) - inv)
+    num = np.where(np.isnan(num), np.inf, num).max(axis=(1, 2))
+    rel = float((num / np.abs(inv).max(axis=(1, 2))).max()) if len(a) else 0.0
+    if flags in (api.PIVOT_PARALLEL, api.PIVOT_SERIAL):
+        mism = int((pivoted_a(a) != piv).any(axis=1).sum())
+    elif flags == api.PIVOT_NONE:
+        mism = int((piv != np.arange(a.shape[-1])).any(axis=1).sum())
+    else:
+        mism = None
+    return rel, mism
 
 
 def _batch_label(batch: int) -> str:
@@ -56,6 +115,8 @@ def run_config(n: int, batch: int, flags: int, runs: int, inputs: str, seed: int
     pristine = DeviceBuffer.from_array(a)
     work = DeviceBuffer(a.nbytes)
     piv = DeviceBuffer(max(batch * n * 4, 4))
+    sample = min(batch, CHECK_SAMPLE)
+    a_sample = a[:sample].copy()
     del a
     e0, e1 = Event(), Event()
     times = []
@@ -73,10 +134,13 @@ def run_config(n: int, batch: int, flags: int, runs: int, inputs: str, seed: int
             correct, worst = api.residual(pristine, work, n, batch, 1e-3, stream)
             bad_counts.append(batch - correct)
             max_res = max(max_res, worst) if worst == worst else float("nan")
+        x_sample = work.download((sample, n, n), np.float32)
+        p_sample = piv.download((sample, n), np.int32)
     finally:
         pristine.close()
         work.close()
         piv.close()
+    rel, mism = host_checks(a_sample, x_sample, p_sample, flags)
     cold, warm = times[0], np.array(times[1:])
     bytes_per = 8 * n * n + (4 * n if flags != api.PIVOT_NONE else 0)
     avg = float(warm.mean())
@@ -96,6 +160,10 @@ def run_config(n: int, batch: int, flags: int, runs: int, inputs: str, seed: int
         "gbps": batch * bytes_per / (avg * 1e-3) / 1e9,
         "pct_hbm": 100.0 * batch * bytes_per / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS,
         "max_abs_AXmI": max_res,
+        "max_rel_err_vs_fp64": rel,
+        "pivot_mismatches": mism,
+        "check_sample": sample,
+        "cpu_cores": cpu_cores(),
         "pivot": [k for k, v in api.PIVOT_NAMES.items() if v == flags][0],
         "inputs": inputs,
     }

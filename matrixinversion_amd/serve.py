@@ -15,6 +15,9 @@ reports latency percentiles and throughput:
   * latency = completion seen by the driver - scheduled arrival (it includes
     any wait for admission under the budget).
 
+``warm`` runs a short untimed stream first, so that the first launches of
+each kernel (code object loading) stay out of the measured one.
+
     python -m matrixinversion_amd.serve --rates 2000,8000,20000 --duration 2
 """
 from __future__ import annotations
@@ -43,6 +46,7 @@ class Server:
               "lu_server_create")
         self.handle = h.value
         self._lib = lib()
+        self._poll_buf = (ctypes.c_uint64 * 4096)()
 
     def submit(self, src_ptr: int, ns: np.ndarray, flags: int):
         """Ticket, or None when the request does not fit right now (LUB_EBUSY)."""
@@ -70,6 +74,23 @@ class Server:
 
     def release(self, ticket: int) -> None:
         check(self._lib.lu_server_release(self.handle, ticket), "lu_server_release")
+
+    def poll(self) -> list:
+        """Tickets finished since the last poll, in completion order."""
+        out = []
+        cnt = ctypes.c_int()
+        while True:
+            check(self._lib.lu_server_poll(self.handle, self._poll_buf, len(self._poll_buf), ctypes.byref(cnt)),
+                  "lu_server_poll")
+            out.extend(self._poll_buf[:cnt.value])
+            if cnt.value < len(self._poll_buf):
+                return out
+
+    def counters(self):
+        """(batches launched, most batches of one pivot rule running at once)."""
+        b, p = ctypes.c_int64(), ctypes.c_int()
+        check(self._lib.lu_server_counters(self.handle, ctypes.byref(b), ctypes.byref(p)), "lu_server_counters")
+        return b.value, p.value
 
     def stats(self):
         q, r, b = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
@@ -150,6 +171,7 @@ def poisson_stream(server: Server, pools, rate: float, duration: float, seed: in
     nxt = 0
     done = 0
     batches = 0
+    b0, _ = server.counters()
     t0 = time.perf_counter()
     while done < n_req:
         now = time.perf_counter() - t0
@@ -166,21 +188,27 @@ def poisson_stream(server: Server, pools, rate: float, duration: float, seed: in
         batches += launched
         if completed:
             now = time.perf_counter() - t0
-            for t in list(inflight):
-                st, _, _, _ = server.query(t)
-                if st == DONE:
-                    i = inflight.pop(t)
-                    lat[i] = (now - arrivals[i]) * 1e3
-                    server.release(t)
-                    done += 1
+            for t in server.poll():
+                i = inflight.pop(t)
+                lat[i] = (now - arrivals[i]) * 1e3
+                server.release(t)
+                done += 1
     wall = time.perf_counter() - t0
+    b1, peak = server.counters()
     mats = int(counts.sum())
     return {"rate_req_s": rate, "requests": n_req, "matrices": mats, "wall_s": wall,
             "matrices_per_s": mats / wall, "requests_per_s": n_req / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "batches": batches,
             "matrices_per_batch": mats / max(batches, 1), "busy_retries": busy_retries,
-            "pivot_share": float(pivot.mean())}
+            "pivot_share": float(pivot.mean()), "peak_rule_inflight_since_create": peak}
+
+
+def warm(server: Server, pools, rate: float = 4000.0, duration: float = 0.15) -> None:
+    """An untimed stream through `server` (both pivot rules): the first launch
+    of each kernel loads its code object, which a measured stream should not
+    see."""
+    poisson_stream(server, pools, rate, duration, seed=1001)
 
 
 def main(argv=None) -> int:
@@ -195,6 +223,7 @@ def main(argv=None) -> int:
     pools = (Pool(42, args.pool), Pool(43, args.pool, dominant=True))
     out = []
     with Server(int(args.budget_gb * 2**30)) as srv:
+        warm(srv, pools)
         for r in (float(x) for x in args.rates.split(",")):
             res = poisson_stream(srv, pools, r, args.duration)
             print(json.dumps(res), flush=True)

@@ -69,11 +69,12 @@ def _load():
         lib.orc_verify_inv.argtypes = [_c_float_p, _c_float_p, ctypes.c_int, ctypes.c_longlong,
                                        ctypes.c_float, _c_double_p]
         lib.orc_verify_inv.restype = ctypes.c_longlong
+        lib.orc_verify_inv_threads.argtypes = [_c_float_p, _c_float_p, ctypes.c_int, ctypes.c_longlong,
+                                               ctypes.c_float, _c_double_p, ctypes.c_int]
+        lib.orc_verify_inv_threads.restype = ctypes.c_longlong
         lib.orc_batch_gepp_f32.argtypes = [_c_float_p, _c_int_p, _c_int_p, ctypes.c_int, ctypes.c_longlong,
                                            ctypes.c_int]
         lib.orc_batch_gepp_f32.restype = ctypes.c_int
-        lib.orc_max_rel_err.argtypes = [_c_float_p, _c_float_p, ctypes.c_int, ctypes.c_longlong]
-        lib.orc_max_rel_err.restype = ctypes.c_double
         _lib = lib
     return _lib
 
@@ -210,23 +211,19 @@ def cond1(a: np.ndarray) -> float:
     return float(_load().orc_cond1(ap, a.shape[-1]))
 
 
-def verify_inv(a: np.ndarray, x: np.ndarray, thr: float = 1e-3):
-    """verifyInv semantics: (correct_count, max|AX - I| in FP64)."""
+def verify_inv(a: np.ndarray, x: np.ndarray, thr: float = 1e-3, threads: int = 1):
+    """verifyInv semantics: (correct_count, max|AX - I| in FP64); threads > 1
+    (or 0 = all) splits the matrices over OpenMP threads."""
     n = a.shape[-1]
     a, ap = _f32(a)
     x, xp = _f32(x)
     batch = a.size // (n * n)
     worst = ctypes.c_double(0.0)
-    correct = _load().orc_verify_inv(ap, xp, n, batch, thr, ctypes.byref(worst))
+    if threads == 1:
+        correct = _load().orc_verify_inv(ap, xp, n, batch, thr, ctypes.byref(worst))
+    else:
+        correct = _load().orc_verify_inv_threads(ap, xp, n, batch, thr, ctypes.byref(worst), threads)
     return int(correct), float(worst.value)
-
-
-def max_rel_err(a: np.ndarray, x: np.ndarray) -> float:
-    """max over the batch of max|X - X64| / max|X64| (X64 = FP64 inverse)."""
-    n = a.shape[-1]
-    a, ap = _f32(a)
-    x, xp = _f32(x)
-    return float(_load().orc_max_rel_err(ap, xp, n, a.size // (n * n)))
 
 
 # ----------------------------------------------- reference's own checker

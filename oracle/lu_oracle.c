@@ -392,9 +392,23 @@ double orc_cond1(const float* A, int n) {
  * matrix counts as correct when all diagonal r are within thr of 1 and all
  * off-diagonal r within thr of 0.  Returns the number of correct inversions;
  * max_abs (may be NULL) receives max |A X - I| over the batch computed in FP64. */
+long long orc_verify_inv_threads(const float* A, const float* X, int n, long long batch, float thr,
+                                 double* max_abs, int threads);
+
 long long orc_verify_inv(const float* A, const float* X, int n, long long batch, float thr, double* max_abs) {
+    return orc_verify_inv_threads(A, X, n, batch, thr, max_abs, 1);
+}
+
+/* The same count over `threads` OpenMP threads (matrices split between them;
+ * each matrix is still checked in the reference's order).  threads <= 0: all. */
+long long orc_verify_inv_threads(const float* A, const float* X, int n, long long batch, float thr,
+                                 double* max_abs, int threads) {
     long long correct = 0;
     double worst = 0.0;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for num_threads(nt) schedule(static) reduction(+ : correct) reduction(max : worst)
+#endif
     for (long long b = 0; b < batch; ++b) {
         const float* a = A + (size_t)b * n * n;
         const float* x = X + (size_t)b * n * n;
@@ -416,27 +430,6 @@ long long orc_verify_inv(const float* A, const float* X, int n, long long batch,
         correct += ok;
     }
     if (max_abs) *max_abs = worst;
+    (void)threads;
     return correct;
-}
-
-/* Max over the batch of max_ij |X - Xref| / max_ij |Xref|, with Xref the FP64
- * inverse of each FP32 input.  Returns -1 if any input is exactly singular. */
-double orc_max_rel_err(const float* A, const float* X, int n, long long batch) {
-    double worst = 0.0;
-    double R[ORC_MAXN * ORC_MAXN];
-    for (long long b = 0; b < batch; ++b) {
-        const float* a = A + (size_t)b * n * n;
-        const float* x = X + (size_t)b * n * n;
-        if (orc_inverse_f64(a, R, n, NULL) != 0) return -1.0;
-        double num = 0.0, den = 0.0;
-        for (int i = 0; i < n * n; ++i) {
-            double d = fabs((double)x[i] - R[i]);
-            if (isnan(d)) d = INFINITY;
-            if (d > num) num = d;
-            if (fabs(R[i]) > den) den = fabs(R[i]);
-        }
-        double e = num / den;
-        if (e > worst) worst = e;
-    }
-    return worst;
 }

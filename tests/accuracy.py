@@ -1,17 +1,35 @@
 """Accuracy helpers shared by the parity tests (CPU side, numpy + oracle).
 
 Error measure, per matrix: e(X) = max_ij |X - X64| / max_ij |X64|, with X64
-the FP64 inverse of the FP32 input (oracle.inverse_f64).  The contract stated
-by the library for FP32 results is
+the FP64 inverse of the FP32 input (oracle.inverse_f64).
 
-    e(X_gpu) <= C_REL * n * kappa_1(A) * 2^-24        (well-conditioned, any pivoting)
+The contract has three parts.
 
-and, for every input (the REF pivot rule is not GEPP, so growth can push
-both the reference's arithmetic and ours past the first bound), that the GPU
-stays within a small factor of the larger of that bound and the error of the
-FP32 restatement of the reference's own arithmetic:
+1. Distribution gate (every value check).  The GPU uses the oracle's pivots
+   (bit-exact, checked separately) and the same elimination multipliers, so
+   its error is rounding noise of the same elimination as the oracle's.  Per
+   test set the ratios r = e(X_gpu) / e(X_oracle) must have
 
-    e(X_gpu) <= C_ORC * max(e(X_oracle), n * kappa_1(A) * 2^-24)
+       median(r) <= R_MED   and   90th percentile(r) <= R_P90
+
+   (errors below E_FLOOR, a quarter ulp of the largest entry, count as
+   E_FLOOR, so two essentially exact results have ratio 1).  The constants
+   were measured once on MI355X against these oracles, per input class, and
+   carry a margin; an inverse whose error is uniformly 10x the oracle's
+   fails them, and so does one 3x worse than the GPU's measured error (CPU
+   tests plant both in the oracle's own result).
+
+2. Diagonally dominant inputs (A = U[-1,1) + n*I):
+
+       e(X_gpu) <= min(n, C_DOM) * kappa_1(A) * 2^-24    and    max|A X - I| <= 1e-4
+
+   (the factor stops growing at C_DOM: the oracle's own worst case over
+   n = 1..32 is 0.98 kappa_1 2^-24, the GPU's 4.0, at n = 32).
+
+3. A per-matrix guard for every input (the reference's pivot rule is not
+   LAPACK's, so growth can push its arithmetic far past n * kappa * 2^-24):
+
+       e(X_gpu) <= C_ORC * max(e(X_oracle), n * kappa_1(A) * 2^-24)
 """
 from __future__ import annotations
 
@@ -20,8 +38,30 @@ import numpy as np
 import oracle
 
 EPS32 = 2.0 ** -24
-C_REL = 1.0      # multiplier of n * kappa_1 * 2^-24 (diagonally dominant inputs)
-C_ORC = 8.0      # GPU vs max(oracle error, n * kappa_1 * 2^-24)
+C_DOM = 8.0      # cap of the multiplier of kappa_1 * 2^-24 (diagonally dominant inputs)
+C_ORC = 8.0      # per-matrix guard: GPU vs max(oracle error, n * kappa_1 * 2^-24)
+E_FLOOR = 2.0 ** -26
+
+# Ratio gates, e_gpu / e_oracle per test set, measured on MI355X (largest
+# median and 90th percentile over every gated set) and set with a margin:
+#
+#   reference pivot rules, U[-1,1) inputs       median 0.50-0.81, p90 1.31-2.00
+#     against the oracle's Doolittle LU + column solves: the GPU's
+#     Gauss-Jordan on the same pivots is the more accurate of the two here
+#   no pivoting, diagonally dominant inputs     median 1.77-2.70, p90 3.24-4.27
+#     both errors are a few ulp; Gauss-Jordan rounds more often per entry
+#   partial pivoting (GEPP)                     median 1.000, p90 1.000
+#     against the oracle's FP32 Gauss-Jordan in the same operation order
+#     (division instead of the GPU's reciprocal + Newton step)
+#
+# Each pair fails an inverse whose errors are uniformly 3x those measured
+# for the GPU in its class (CPU tests plant such inverses in the oracle's).
+R_MED = 1.5
+R_P90 = 3.0
+R_MED_DOM = 4.0
+R_P90_DOM = 6.0
+R_MED_GEPP = 1.25
+R_P90_GEPP = 1.5
 
 
 def rel_err64(a: np.ndarray, x: np.ndarray) -> np.ndarray:
@@ -58,3 +98,60 @@ def contract_bound(a: np.ndarray, e_oracle: np.ndarray) -> np.ndarray:
     """Per-matrix C_ORC * max(e_oracle, n * kappa_1 * 2^-24)."""
     n = a.shape[-1]
     return C_ORC * np.maximum(e_oracle, n * kappa1(a) * EPS32)
+
+
+def dominant_bound(a: np.ndarray) -> np.ndarray:
+    """Per-matrix min(n, C_DOM) * kappa_1 * 2^-24 (diagonally dominant inputs)."""
+    return min(a.shape[-1], C_DOM) * kappa1(a) * EPS32
+
+
+def error_ratios(e_gpu: np.ndarray, e_orc: np.ndarray) -> np.ndarray:
+    """Per-matrix e_gpu / e_oracle with both floored at E_FLOOR; matrices
+    where either error is not finite are left out (the per-matrix guard
+    covers them)."""
+    e_gpu = np.asarray(e_gpu, dtype=np.float64).ravel()
+    e_orc = np.asarray(e_orc, dtype=np.float64).ravel()
+    ok = np.isfinite(e_gpu) & np.isfinite(e_orc)
+    return np.maximum(e_gpu[ok], E_FLOOR) / np.maximum(e_orc[ok], E_FLOOR)
+
+
+def ratio_stats(e_gpu, e_orc) -> tuple[float, float, float, int]:
+    """(median, 90th percentile, max, count) of error_ratios."""
+    r = error_ratios(e_gpu, e_orc)
+    if r.size == 0:
+        return 1.0, 1.0, 1.0, 0
+    return float(np.median(r)), float(np.percentile(r, 90)), float(r.max()), int(r.size)
+
+
+def ratio_gate_ok(e_gpu, e_orc, med: float = R_MED, p90: float = R_P90) -> bool:
+    m, p, _, cnt = ratio_stats(e_gpu, e_orc)
+    return cnt > 0 and m <= med and p <= p90
+
+
+class RatioGate:
+    """Collects (e_gpu, e_oracle) over a test set; check() applies the
+    distribution gate to all of them at once and prints the statistics.
+    kind: "ref" (the reference's pivot rules), "dominant" (no pivoting on
+    diagonally dominant inputs) or "gepp"."""
+
+    LIMITS = {"ref": (R_MED, R_P90), "dominant": (R_MED_DOM, R_P90_DOM), "gepp": (R_MED_GEPP, R_P90_GEPP)}
+
+    def __init__(self, tag: str, kind: str = "ref"):
+        self.tag = tag
+        self.med, self.p90 = self.LIMITS[kind]
+        self.g: list[np.ndarray] = []
+        self.o: list[np.ndarray] = []
+
+    def add(self, e_gpu, e_orc) -> None:
+        self.g.append(np.asarray(e_gpu, dtype=np.float64).ravel())
+        self.o.append(np.asarray(e_orc, dtype=np.float64).ravel())
+
+    def check(self) -> tuple[float, float]:
+        e_gpu, e_orc = np.concatenate(self.g), np.concatenate(self.o)
+        m, p, mx, cnt = ratio_stats(e_gpu, e_orc)
+        print(f"[gate] {self.tag}: e_gpu/e_oracle over {cnt} matrices: median {m:.3f} "
+              f"(<= {self.med}), p90 {p:.3f} (<= {self.p90}), max {mx:.3f}", flush=True)
+        assert cnt > 0, f"{self.tag}: no finite errors to gate"
+        assert m <= self.med, f"{self.tag}: median e_gpu/e_oracle {m:.3f} > {self.med}"
+        assert p <= self.p90, f"{self.tag}: p90 e_gpu/e_oracle {p:.3f} > {self.p90}"
+        return m, p

@@ -145,7 +145,19 @@ def test_server_validation_codes():
     assert lib.lu_server_submit(None, 0x1000, 0x1000, 1, 1, ctypes.byref(t)) == 3
     assert lib.lu_server_progress(None, None, None) == 3
     assert lib.lu_server_release(None, 1) == 3
+    assert lib.lu_server_poll(None, None, 0, None) == 3
+    cnt = ctypes.c_int()
+    assert lib.lu_server_poll(None, None, 0, ctypes.byref(cnt)) == 3
+    assert lib.lu_server_counters(None, None, None) == 3
     assert lib.lu_status_string(9).decode().startswith("no room")
+
+
+def test_build_id_is_the_source_hash():
+    """lu_build_id() of the loaded library is the hash of the sources in this
+    tree (so counter profiles can be matched to the code they measured)."""
+    from matrixinversion_amd import _native, build
+
+    assert _native.lib().lu_build_id().decode() == build.source_id()
 
 
 def test_gather_scatter_validation_codes():

@@ -75,7 +75,11 @@ def test_headline_configuration_full_size(gpu):
         assert (piv[sample] == oracle.batch_pivots(block[sample], oracle.PIV_FIRSTMAX)).all()
         xo, _ = oracle.lu_inverse_f32(block[sample], oracle.PIV_FIRSTMAX)
         e = acc.rel_err64(block[sample], x[sample])
-        assert np.all(e <= acc.contract_bound(block[sample], acc.rel_err64(block[sample], xo)))
+        e_orc = acc.rel_err64(block[sample], xo)
+        assert np.all(e <= acc.contract_bound(block[sample], e_orc))
+        gate = acc.RatioGate("headline sample of 512")
+        gate.add(e, e_orc)
+        gate.check()
         correct, _ = gpu.residual(A, X, n, BATCH, 1e-3)
         correct_block = oracle.verify_inv(block, x)[0]
         correct_tail = oracle.verify_inv(block[:tail], x[:tail])[0]

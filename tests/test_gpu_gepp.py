@@ -5,7 +5,8 @@ reference, whose rules search the un-updated column.
 Its pivot vectors must equal, bit for bit, those of the oracle's FP32 GEPP
 restatement (oracle.gepp_inverse_f32), of LAPACK dgetrf (scipy) and of
 rocSOLVER sgetrf on the device.  Its inverses must meet the accuracy
-contract of tests/accuracy.py against that restatement.  On U[-1,1] inputs,
+contract of tests/accuracy.py against that restatement (the per-matrix
+guard and the distribution gate with the partial-pivoting constants).  On U[-1,1] inputs,
 where the reference rule loses accuracy, it must be clearly more accurate.
 Every case prints its name before touching the GPU.
 """
@@ -23,7 +24,7 @@ GEPP = 0x4
 LAYOUTS = {"row16": (0x10, 32), "quad4": (0x20, 16)}
 
 
-def _check(mia, a, tag, layout_flag=0):
+def _check(mia, a, tag, layout_flag=0, gate=None):
     print(f"[case] gepp {tag} shape={a.shape}", flush=True)
     x, piv = mia.invert(a, GEPP | layout_flag)
     xo, po, info = oracle.gepp_inverse_f32(a)
@@ -35,22 +36,33 @@ def _check(mia, a, tag, layout_flag=0):
     worst = int(np.argmax(e_gpu / bound))
     assert np.all(e_gpu <= bound), (f"{tag}: matrix {worst} e_gpu={e_gpu[worst]:.3e} "
                                     f"e_oracle={e_orc[worst]:.3e} bound={bound[worst]:.3e}")
+    if gate is not None:
+        gate.add(e_gpu, e_orc)
     return x, piv, e_gpu, e_orc
+
+
+def _gate(tag):
+    return acc.RatioGate(tag, "gepp")
 
 
 @pytest.mark.parametrize("layout", ["row16", "quad4"])
 def test_gepp_all_sizes(gpu, layout):
     """Every n of the layout; 37 matrices end in a partial tile in both."""
     lf, nmax = LAYOUTS[layout]
+    gate = _gate(f"gepp all sizes {layout}")
     for n in range(1, nmax + 1):
         a = oracle.uniform_matrices(3000 + n, 37, n)
-        _check(gpu, a, f"n={n} {layout}", lf)
+        _check(gpu, a, f"n={n} {layout}", lf, gate)
+    gate.check()
 
 
 @pytest.mark.parametrize("batch", [1, 3, 5, 17, 1000])
 def test_gepp_batch_tails_n32(gpu, batch):
     a = oracle.uniform_matrices(3100 + batch, batch, 32)
-    _check(gpu, a, f"batch={batch}")
+    gate = _gate(f"gepp n=32 batch={batch}")
+    _check(gpu, a, f"batch={batch}", gate=gate)
+    if batch >= 100:
+        gate.check()
 
 
 def test_gepp_ipiv_and_info_equal_lapack_getrf(gpu):

@@ -1,7 +1,8 @@
 """GPU parity: liblubatch (through the C ABI) against the CPU oracle.
 
 Pivot vectors must match the oracle bit for bit; inverses must meet the FP32
-accuracy contract of tests/accuracy.py.  Inputs are seeded mt19937 streams
+accuracy contract of tests/accuracy.py: per matrix the guard bound, and per
+test set the distribution gate on e_gpu / e_oracle.  Inputs are seeded mt19937 streams
 (oracle.uniform_matrices / dominant_matrices) at sizes the oracle finishes in
 seconds.  Every case prints its name before touching the GPU.
 """
@@ -30,7 +31,7 @@ def _inputs(mode, n, batch, seed):
 LAYOUTS = {"row16": (0x10, 32), "quad4": (0x20, 16)}
 
 
-def _check(mia, a, mode, tag, layout_flag=0):
+def _check(mia, a, mode, tag, layout_flag=0, gate=None):
     flags, rule = MODES[mode]
     flags |= layout_flag
     print(f"[case] {tag} mode={mode} shape={a.shape}", flush=True)
@@ -44,6 +45,8 @@ def _check(mia, a, mode, tag, layout_flag=0):
     worst = int(np.argmax(e_gpu / bound))
     assert np.all(e_gpu <= bound), (f"{tag}: matrix {worst} e_gpu={e_gpu[worst]:.3e} "
                                     f"e_oracle={e_orc[worst]:.3e} bound={bound[worst]:.3e}")
+    if gate is not None:
+        gate.add(e_gpu, e_orc)
     return x, piv, e_gpu, e_orc
 
 
@@ -54,34 +57,44 @@ def test_all_sizes(gpu, mode, layout):
     (9 tiles + 1 of 4, or 2 tiles + 5 of 16)."""
     lf, nmax = LAYOUTS[layout]
     rows = []
+    gate = acc.RatioGate(f"all sizes mode={mode} {layout}", "dominant" if mode == "none" else "ref")
     for n in range(1, nmax + 1):
         a = _inputs(mode, n, 37, seed=1000 + n)
-        _, _, e_gpu, e_orc = _check(gpu, a, mode, f"n={n} {layout}", lf)
+        _, _, e_gpu, e_orc = _check(gpu, a, mode, f"n={n} {layout}", lf, gate)
         rows.append((n, e_gpu.max(), e_orc.max()))
     print("\n".join(f"  n={n:2d} e_gpu={g:.2e} e_orc={o:.2e}" for n, g, o in rows))
+    gate.check()
 
 
 @pytest.mark.parametrize("batch", [1, 2, 3, 4, 5, 7, 8, 9, 63, 64, 65, 1000])
 def test_batch_tails_n32(gpu, batch):
     a = oracle.uniform_matrices(42, batch, 32)
-    _check(gpu, a, "parallel", f"batch={batch}")
+    gate = acc.RatioGate(f"n=32 batch={batch}")
+    _check(gpu, a, "parallel", f"batch={batch}", gate=gate)
+    if batch >= 100:
+        gate.check()
 
 
 @pytest.mark.parametrize("batch", [1, 15, 16, 17, 33, 1000])
 def test_batch_tails_quads(gpu, batch):
+    gate = acc.RatioGate(f"quad4 batch={batch}")
     for n in (4, 7, 16):
         a = oracle.uniform_matrices(43, batch, n)
-        _check(gpu, a, "parallel", f"batch={batch} n={n} quad4", 0x20)
+        _check(gpu, a, "parallel", f"batch={batch} n={n} quad4", 0x20, gate)
+    if batch >= 100:
+        gate.check()
 
 
 def test_config1_n32_1000(gpu):
     """1,000 x n=32, U[-1,1] from mt19937(42), pivoted: the reference's C1 case."""
     a = oracle.uniform_matrices(42, 1000, 32)
-    x, piv, e_gpu, e_orc = _check(gpu, a, "parallel", "C1")
+    gate = acc.RatioGate("C1 1000 x n=32")
+    x, piv, e_gpu, e_orc = _check(gpu, a, "parallel", "C1", gate=gate)
     correct, worst = oracle.verify_inv(a, x)
     correct_o, worst_o = oracle.verify_inv(a, oracle.lu_inverse_f32(a, oracle.PIV_FIRSTMAX)[0])
     print(f"  C1 verifyInv correct gpu={correct} oracle={correct_o}; max|AX-I| gpu={worst:.3e} oracle={worst_o:.3e}; "
           f"median e_gpu={np.median(e_gpu):.2e} e_orc={np.median(e_orc):.2e}")
+    gate.check()
 
 
 def test_ties_use_exact_rule(gpu):
@@ -112,8 +125,10 @@ def test_real_valued_ties_of_the_1m_workload(gpu):
     from tests.conftest import GOLDEN
 
     a = np.load(os.path.join(GOLDEN, "seed42_tie_matrices.npz"))["a"]
+    gate = acc.RatioGate("seed42 1M ties")
     for mode in ("parallel", "serial", "tree"):
-        _check(gpu, a, mode, "seed42 1M ties")
+        _check(gpu, a, mode, "seed42 1M ties", gate=gate)
+    gate.check()
 
 
 def test_nan_and_zero_pivot_rules(gpu):
@@ -162,18 +177,26 @@ def test_unaligned_batch(gpu):
 
 @pytest.mark.parametrize("mode", ["none", "parallel"])
 def test_well_conditioned_bounds(gpu, mode):
-    """Diagonally dominant inputs (A = U[-1,1] + n*I): e <= n*kappa_1*2^-24 and
-    max|A X - I| <= 1e-4 for every n."""
-    flags, _ = MODES[mode]
+    """Diagonally dominant inputs (A = U[-1,1] + n*I): e <= min(n, C_DOM)*kappa_1*2^-24
+    and max|A X - I| <= 1e-4 for every n, and the distribution gate against
+    the oracle over all of them."""
+    flags, rule = MODES[mode]
+    gate = acc.RatioGate(f"dominant mode={mode}", "dominant")
+    worst = 0.0
     for n in range(1, 33):
         a = oracle.dominant_matrices(2000 + n, 64, n)
         print(f"[case] dominant n={n} mode={mode}", flush=True)
         x = gpu.invert(a, flags, return_pivots=False)
         e = acc.rel_err64(a, x)
-        b = acc.C_REL * n * acc.kappa1(a) * acc.EPS32
+        b = acc.dominant_bound(a)
+        worst = max(worst, float((e / (acc.kappa1(a) * acc.EPS32)).max()))
         assert np.all(e <= b), f"n={n}: e={e.max():.3e} bound={b[np.argmax(e / b)]:.3e}"
         r = acc.max_abs_axmi(a, x)
         assert r.max() <= 1e-4, f"n={n}: max|AX-I|={r.max():.3e}"
+        gate.add(e, acc.rel_err64(a, oracle.lu_inverse_f32(a, rule)[0]))
+    print(f"[gate] dominant mode={mode}: max e / (kappa_1 2^-24) = {worst:.3f} (<= min(n, {acc.C_DOM}))",
+          flush=True)
+    gate.check()
 
 
 @pytest.mark.parametrize("n,batch", [(16, 64), (32, 4096)])

@@ -107,6 +107,8 @@ def test_ragged_matches_plain_and_oracle(gpu, mode, layout):
         mask[o:o + n * n] = False
     assert _same_bits(x[mask], flat[mask]), "ragged kernel wrote outside its matrices"
 
+    kind = {"gepp": "gepp", "none": "dominant"}.get(mode, "ref")
+    gate = acc.RatioGate(f"ragged mode={mode} layout={layout:#x}", kind)
     for n in np.unique(ns32):
         sel = np.nonzero(ns32 == n)[0]
         a = np.stack([mats[i] for i in sel])
@@ -122,8 +124,11 @@ def test_ragged_matches_plain_and_oracle(gpu, mode, layout):
             xo, po = oracle.lu_inverse_f32(a, rule)
         assert (pr == po).all(), f"n={n}: pivots differ from the oracle"
         e = acc.rel_err64(a, xr)
-        bound = acc.contract_bound(a, acc.rel_err64(a, xo))
+        e_orc = acc.rel_err64(a, xo)
+        bound = acc.contract_bound(a, e_orc)
         assert np.all(e <= bound), f"n={n}: accuracy contract"
+        gate.add(e, e_orc)
+    gate.check()
 
 
 def test_ragged_invalid_sizes_are_skipped_and_counted(gpu):

@@ -45,8 +45,9 @@ def test_matches_rocsolver_on_well_conditioned(gpu, n):
     xr, info = _rocsolver_inverse(gpu, a)
     assert (info == 0).all()
     xo, _ = gpu.invert(a, 0x1)
-    bound = acc.C_REL * n * acc.kappa1(a) * 2.0 ** -24
-    assert np.all(acc.rel_err64(a, xr) <= bound)
-    assert np.all(acc.rel_err64(a, xo) <= bound)
+    # rocSOLVER (getrf + getri) within the textbook n * kappa_1 * 2^-24; ours
+    # within the library's own tighter contract for dominant inputs
+    assert np.all(acc.rel_err64(a, xr) <= n * acc.kappa1(a) * acc.EPS32)
+    assert np.all(acc.rel_err64(a, xo) <= acc.dominant_bound(a))
     scale = np.abs(xr).max(axis=(1, 2), keepdims=True)
     assert np.all(np.abs(xo - xr) <= 1e-5 * scale)

@@ -117,13 +117,61 @@ def test_budget_backpressure(gpu):
 
 
 def test_poisson_stream_short(gpu):
+    """A warmed-up server keeps the 99th percentile of end-to-end latency
+    under 1 ms at 2000 requests/s (1..4096 matrices each, n in 4..32)."""
     from matrixinversion_amd import serve
 
     pools = (serve.Pool(42, 20000), serve.Pool(43, 20000, dominant=True))
-    print("[case] serve poisson 2000 req/s for 0.3 s", flush=True)
+    print("[case] serve poisson 2000 req/s for 0.5 s", flush=True)
     with serve.Server(4 << 30) as srv:
-        res = serve.poisson_stream(srv, pools, 2000.0, 0.3)
+        serve.warm(srv, pools)
+        res = serve.poisson_stream(srv, pools, 2000.0, 0.5)
     print(res, flush=True)
-    assert res["requests"] > 0 and np.isfinite(res["p99_ms"]) and res["p50_ms"] > 0
+    assert res["requests"] > 0 and res["p50_ms"] > 0
+    assert res["p99_ms"] <= 1.0, f"p99 {res['p99_ms']:.3f} ms at 2000 req/s"
     for p in pools:
         p.close()
+
+
+def test_same_rule_batches_overlap(gpu):
+    """A small request that arrives while a large batch of the same pivot rule
+    runs is launched at once on another slot (two batches of one rule in
+    flight), and both results are bit-identical to inverting them alone."""
+    from matrixinversion_amd import serve
+
+    big = serve.Pool(46, 400_000, 32, 32)
+    small = serve.Pool(47, 2000)
+    print("[case] serve same-rule overlap: 400k x n=32 then 300 mixed", flush=True)
+    with serve.Server(8 << 30) as srv:
+        p1, ns1 = big.request(0, 400_000)
+        t1 = srv.submit(p1, ns1, 0x1)
+        assert t1 is not None
+        launched, _ = srv.progress()
+        assert launched == 1
+        p2, ns2 = small.request(100, 300)
+        t2 = srv.submit(p2, ns2, 0x1)
+        assert t2 is not None
+        launched, _ = srv.progress()
+        assert launched == 1, "the second request waited for the first batch"
+        assert srv.counters()[1] >= 2
+        order = []
+        for _ in range(10_000_000):
+            srv.progress()
+            order.extend(srv.poll())
+            if len(order) == 2:
+                break
+        assert sorted(order) == sorted([t1, t2])
+        print(f"  completion order {order} (first submitted {t1})", flush=True)
+        _check_request(gpu, serve, srv, t2, small, 100, 300, 0x1)
+        st, res, piv, _ = srv.query(t1)
+        assert st == serve.DONE
+        host = big.host(0, 400_000)
+        x = serve.download(res, host.nbytes, np.float32)
+        xa, pa = _alone(gpu, host, ns1, 0x1)
+        assert np.array_equal(x.view(np.uint32), xa.view(np.uint32)), "large request differs from alone"
+        pv = serve.download(piv, int(ns1.sum()) * 4, np.int32)
+        assert np.array_equal(pv, np.concatenate(pa))
+        srv.release(t1)
+        srv.release(t2)
+    big.close()
+    small.close()

@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.conftest import GOLDEN
+from tests.conftest import GOLDEN, ROOT
 
 
 def _fixture(name):
@@ -324,6 +324,22 @@ def test_verify_inv_counts_match_reference():
         correct, _ = oracle.verify_inv(a, x)
         assert correct == case["correct"]
         assert case["correct"] == 2 and case["wrong"] == 4
+        assert oracle.verify_inv(a, x, threads=0)[0] == case["correct"]
+
+
+def test_threaded_verify_inv_equals_single_threaded():
+    """The OpenMP form (timed for the CPU baseline) counts exactly what the
+    single-threaded restatement counts, on inverses perturbed across 1e-3."""
+    a = oracle.uniform_matrices(31, 600, 16)
+    x, _ = oracle.lu_inverse_f32(a, oracle.PIV_FIRSTMAX)
+    rng = np.random.RandomState(2)
+    x[rng.choice(600, 200, replace=False), 3, 4] += np.float32(2e-3)
+    x[5] = np.nan
+    c1, w1 = oracle.verify_inv(a, x)
+    for t in (0, 3):
+        ct, wt = oracle.verify_inv(a, x, threads=t)
+        assert ct == c1 and (wt == w1 or (np.isinf(wt) and np.isinf(w1)))
+    assert 300 < c1 < 500
 
 
 def test_cond1_against_published_and_lapack():
@@ -429,3 +445,25 @@ def test_gepp_ties_go_to_the_first_position():
     assert (oracle.ipiv_from_perm(piv[0]) == ipiv + 1).all()
     info = oracle.gepp_inverse_f32(np.zeros((1, 3, 3), np.float32))[2]
     assert info[0] == 1
+
+
+def test_oracle_under_address_and_undefined_sanitizers():
+    """oracle/Makefile's sanitize target builds lu_oracle.c with
+    AddressSanitizer and UndefinedBehaviorSanitizer (host code) into a driver
+    that calls every oracle function, OpenMP batch forms included, on random,
+    tied, NaN and zero inputs for n = 1..32; any report fails the run."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    here = os.path.join(ROOT, "oracle")
+    b = subprocess.run(["make", "-s", "-C", here, "sanitize"], capture_output=True, text=True)
+    if b.returncode != 0 and "asan" in (b.stderr + b.stdout).lower():
+        pytest.skip("sanitizer runtime not installed")
+    assert b.returncode == 0, b.stderr
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run([os.path.join(here, "_san", "oracle_san")], capture_output=True, text=True, env=env,
+                       timeout=300)
+    assert r.returncode == 0 and "san_driver: ok" in r.stdout, r.stderr[-3000:]
+    assert "runtime error" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr
