@@ -41,6 +41,8 @@ extern "C" {
 #define LUB_EINVAL_ARG    8   /* other invalid argument (ragged tables, ...)  */
 #define LUB_EBUSY         9   /* server: no room under the budget right now;
                                  retry after lu_server_progress / _release    */
+#define LUB_ETOOBIG      10   /* batch above a fixed capacity
+                                 (lu_batched_inverse_ragged_noalloc)          */
 
 #define LUB_MAX_N 32
 
@@ -121,6 +123,20 @@ int lu_batched_inverse_ragged_plan(lub_ragged_plan* plan, float* A, const int64_
  * thread share that plan: issue them on one stream. */
 int lu_batched_inverse_ragged(float* A, const int64_t* offsets, const int* ns, int* piv, const int64_t* piv_offsets,
                               int batch, int flags, void* stream);
+
+/* The same ragged inversion with no allocation at all, for callers that
+ * cannot call hipMalloc (code under stream capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.  The
+ * planning workspace is a fixed device array inside the library, one per GPU,
+ * sized for LUB_NOALLOC_MAX_BATCH matrices; a larger batch returns
+ * LUB_ETOOBIG before any device work.  Every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU (other streams, other host
+ * threads) race: serialise them, e.g. on one stream.  The first call on a
+ * device looks up the array's address (no allocation); afterwards the call
+ * only enqueues kernels and never synchronises. */
+#define LUB_NOALLOC_MAX_BATCH 1048576
+int lu_batched_inverse_ragged_noalloc(float* A, const int64_t* offsets, const int* ns, int* piv,
+                                      const int64_t* piv_offsets, int64_t batch, int flags, void* stream);
 
 /* Matrices of the last enqueued batch whose n was outside 1..32 (waits for it). */
 int lu_ragged_invalid_count(const lub_ragged_plan* plan, int* out);

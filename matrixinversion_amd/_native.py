@@ -28,6 +28,7 @@ ABI = {
     "lu_batched_inverse_ragged_plan": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i, _vp]),
     "lu_batched_inverse_ragged": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "lu_ragged_invalid_count": (_i, [_vp, ctypes.POINTER(_i)]),
+    "lu_batched_inverse_ragged_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _i, _vp]),
     "lu_tuning_set": (_i, [ctypes.c_char_p, ctypes.c_char_p]),
     "lu_batched_refine": (_i, [_vp, _vp, _i, _i64, _vp]),
     "lu_batched_inverse_lapack": (_i, [_vp, _vp, _vp, _i, _i64, _i, _vp]),

@@ -239,6 +239,7 @@ const char* lu_status_string(int status) {
     case LUB_EALIGN: return "pointer not 4-byte aligned";
     case LUB_EINVAL_ARG: return "invalid argument";
     case LUB_EBUSY: return "no room under the server budget now";
+    case LUB_ETOOBIG: return "batch above the entry point's fixed capacity";
     default: return "unknown status";
   }
 }

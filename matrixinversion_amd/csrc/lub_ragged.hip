@@ -284,6 +284,45 @@ int enqueue_ragged(char* ws, int64_t capacity, int cus, float* A, const int64_t*
   return hipGetLastError() == hipSuccess ? LUB_OK : LUB_EHIP;
 }
 
+// Scratch of lu_batched_inverse_ragged_noalloc: the planning workspace for
+// batches of up to LUB_NOALLOC_MAX_BATCH matrices (about 16.1 MiB), one copy
+// per device for the life of the process.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol):
+// the device address comes from hipGetSymbolAddress, once per device.
+constexpr int64_t kNoAllocMax = LUB_NOALLOC_MAX_BATCH;
+constexpr int64_t kNoAllocBytes = lub::RaggedWs::bytes(kNoAllocMax);
+__device__ __align__(256) char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+  std::once_flag once;
+  char* ptr = nullptr;  // device address of `scratch` on this device
+  int cus = 0;
+  int status = LUB_ENODEVICE;
+};
+NoAllocDevice g_noalloc[kMaxDevices];
+
+// Looks up this device's scratch address (and compute-unit count) on the
+// first call made with the device current; later calls read the cache.
+const NoAllocDevice* noalloc_device(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  NoAllocDevice& d = g_noalloc[dev];
+  std::call_once(d.once, [&d, dev] {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) {
+      d.status = LUB_EHIP;
+      return;
+    }
+    if (hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      d.status = LUB_EHIP;
+      return;
+    }
+    d.ptr = static_cast<char*>(p);
+    d.status = LUB_OK;
+  });
+  return &d;
+}
+
 struct ThreadPlan {
   lub_ragged_plan* plan = nullptr;
   ~ThreadPlan() {
@@ -370,6 +409,20 @@ int lu_batched_inverse_ragged(float* A, const int64_t* offsets, const int* ns, i
     tp.plan = p;
   }
   return lu_batched_inverse_ragged_plan(tp.plan, A, offsets, ns, piv, piv_offsets, batch, flags, stream);
+}
+
+int lu_batched_inverse_ragged_noalloc(float* A, const int64_t* offsets, const int* ns, int* piv,
+                                      const int64_t* piv_offsets, int64_t batch, int flags, void* stream) {
+  const int rc = check_ragged_args(A, offsets, ns, piv, piv_offsets, batch, flags);
+  if (rc >= 0) return rc;
+  if (batch > kNoAllocMax) return LUB_ETOOBIG;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return LUB_EHIP;
+  const NoAllocDevice* d = noalloc_device(dev);
+  if (d == nullptr) return LUB_ENODEVICE;
+  if (d->status != LUB_OK) return d->status;
+  return enqueue_ragged(d->ptr, kNoAllocMax, d->cus, A, offsets, ns, piv, piv_offsets, batch, flags,
+                        static_cast<hipStream_t>(stream));
 }
 
 int lu_ragged_invalid_count(const lub_ragged_plan* plan, int* out) {

@@ -150,6 +150,7 @@ def test_server_validation_codes():
     assert lib.lu_server_poll(None, None, 0, ctypes.byref(cnt)) == 3
     assert lib.lu_server_counters(None, None, None) == 3
     assert lib.lu_status_string(9).decode().startswith("no room")
+    assert lib.lu_status_string(10).decode().startswith("batch above")
 
 
 def test_build_id_is_the_source_hash():
