@@ -444,65 +444,94 @@ def bench_refine(args) -> int:
     """Precision workload: 100,000 matrices of n=32, A = Q1 diag(sigma) Q2^T,
     sigma log-spaced 1..10^k for k in {2, 4, 6}; plain FP32 LU inverse vs one
     refinement step (FP64 residual, FP32 MFMA correction), relative error
-    against the FP64 inverse of the same FP32 A.  One JSON line; value = the
-    median relative error after refinement at k = 4."""
+    against the FP64 inverse of the same FP32 A.  Timed: the plain inverse
+    (in place), the fused refined inverse (lu_batched_inverse_refined: A read
+    once, X written once) and the separate refinement pass over A and X.
+    One JSON line; value = the median relative error after refinement at
+    k = 4."""
     import matrixinversion_amd as mia
     from matrixinversion_amd import workloads
+    from matrixinversion_amd._native import check, lib
 
     batch, n = args.batch or 100_000, 32
     stream = mia.Stream()
+    steps = max(2, args.steps)
     per_k = {}
+
+    def per_launch(fn, before=None):
+        """Mean ms of `steps` launches of fn, each bracketed by an event pair
+        on the stream (`before`, untimed, runs ahead of each)."""
+        ev = [mia.Event() for _ in range(2 * steps)]
+        for _ in range(max(1, args.warmup)):
+            if before:
+                before()
+            fn()
+        for i in range(steps):
+            if before:
+                before()
+            ev[2 * i].record(stream)
+            fn()
+            ev[2 * i + 1].record(stream)
+        stream.synchronize()
+        return float(np.mean([ev[2 * i].elapsed_ms(ev[2 * i + 1]) for i in range(steps)]))
+
     for k in (2, 4, 6):
         t0 = time.perf_counter()
         a = workloads.conditioned_matrices(100 + k, batch, n, k)
         log(f"[refine] k={k}: generated {batch} matrices in {time.perf_counter() - t0:.1f}s")
         A = mia.DeviceBuffer.from_array(a)
         X = mia.DeviceBuffer(a.nbytes)
+        X2 = mia.DeviceBuffer(a.nbytes)
         P = mia.DeviceBuffer(batch * n * 4)
-        e = [mia.Event() for _ in range(3)]
-        t_inv, t_ref = [], []
-        for _ in range(max(1, args.steps)):
-            X.copy_from(A, stream)
-            e[0].record(stream)
-            mia.lu_batched_inverse(X, n, batch, mia.PIVOT_PARALLEL, P, stream)
-            e[1].record(stream)
-            if _ == 0:
-                stream.synchronize()
-                x0 = X.download(a.shape, np.float32)
-                e[1].record(stream)
-            mia.refine(A, X, n, batch, stream)
-            e[2].record(stream)
-            stream.synchronize()
-            t_inv.append(e[0].elapsed_ms(e[1]) if _ else float("nan"))
-            t_ref.append(e[1].elapsed_ms(e[2]))
+
+        def restore():
+            mia.device.copy_kernel(X, A, batch * n * n, stream)
+
+        t_inv = per_launch(lambda: mia.lu_batched_inverse(X, n, batch, mia.PIVOT_PARALLEL, P, stream), restore)
+        stream.synchronize()
+        x0 = X.download(a.shape, np.float32)
+        t_ref = per_launch(lambda: mia.refine(A, X, n, batch, stream))   # X: the plain inverse, refined again each call
+        restore()
+        mia.lu_batched_inverse(X, n, batch, mia.PIVOT_PARALLEL, P, stream)
+        mia.refine(A, X, n, batch, stream)
+        stream.synchronize()
         x1 = X.download(a.shape, np.float32)
-        for b in (A, X, P):
+        t_fused = per_launch(lambda: check(lib().lu_batched_inverse_refined(
+            A.ptr, X2.ptr, P.ptr, n, batch, mia.PIVOT_PARALLEL, stream.handle), "inverse_refined"))
+        x2 = X2.download(a.shape, np.float32)
+        for b in (A, X, X2, P):
             b.close()
         inv = np.linalg.inv(a.astype(np.float64))
         nrm = np.linalg.norm(inv, axis=(1, 2))
         e0 = np.linalg.norm(x0 - inv, axis=(1, 2)) / nrm
         e1 = np.linalg.norm(x1 - inv, axis=(1, 2)) / nrm
-        ref_ms = float(np.mean(t_ref))
+        e2 = np.linalg.norm(x2 - inv, axis=(1, 2)) / nrm
         per_k[k] = {"median_rel_err_plain": float(np.median(e0)), "median_rel_err_refined": float(np.median(e1)),
+                    "median_rel_err_fused": float(np.median(e2)),
                     "p99_rel_err_plain": float(np.percentile(e0, 99)),
                     "p99_rel_err_refined": float(np.percentile(e1, 99)),
-                    "refine_ms": ref_ms, "inverse_ms": float(np.nanmean(t_inv)) if len(t_inv) > 1 else None,
-                    "refine_GBps": batch * 12 * n * n / (ref_ms * 1e-3) / 1e9}
-        log(f"[refine] k={k}: median rel err {np.median(e0):.2e} -> {np.median(e1):.2e}, "
-            f"p99 {np.percentile(e0, 99):.2e} -> {np.percentile(e1, 99):.2e}; refine {ref_ms:.3f} ms")
+                    "p99_rel_err_fused": float(np.percentile(e2, 99)),
+                    "inverse_ms": t_inv, "refine_pass_ms": t_ref, "fused_refined_inverse_ms": t_fused,
+                    "fused_over_plain": t_fused / t_inv,
+                    "refine_pass_GBps": batch * 12 * n * n / (t_ref * 1e-3) / 1e9,
+                    "fused_GBps": batch * (8 * n * n + 4 * n) / (t_fused * 1e-3) / 1e9}
+        log(f"[refine] k={k}: median rel err {np.median(e0):.2e} -> {np.median(e1):.2e} (fused {np.median(e2):.2e}), "
+            f"p99 {np.percentile(e0, 99):.2e} -> {np.percentile(e1, 99):.2e}; inverse {t_inv:.3f} ms, "
+            f"fused inverse+refine {t_fused:.3f} ms ({t_fused / t_inv:.2f}x), refinement pass {t_ref:.3f} ms")
+    fb = per_k[4]["fused_GBps"]
     out = {
         "metric": "relative error vs FP64 inverse after one refinement step (median, k=4)",
-        "value": per_k[4]["median_rel_err_refined"],
+        "value": per_k[4]["median_rel_err_fused"],
         "unit": "relative Frobenius error",
-        "n_gpus": 1, "steps": args.steps, "warmup": 0,
-        "ms_per_step": per_k[4]["refine_ms"],
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": per_k[4]["fused_refined_inverse_ms"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 residual)",
         "data": "synthetic: A = Q1 diag(logspace(0,k,32)) Q2^T, Q from QR of RandomState(100+k) Gaussians",
         "config": {"workload": f"refinement, {batch} x n=32, k in (2, 4, 6), pivot=parallel", "batch": batch},
         "detail": per_k,
-        # the refinement kernel: 12 n^2 algorithmic bytes per matrix (A and X read, X written)
-        "roofline": {"bound": "hbm", "achieved": per_k[4]["refine_GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": per_k[4]["refine_GBps"] / HBM_PEAK_GBPS, "traffic": None},
+        # the fused kernel: 8 n^2 + 4 n algorithmic bytes per matrix (A read, X and pivots written)
+        "roofline": {"bound": "hbm", "achieved": fb, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": fb / HBM_PEAK_GBPS, "traffic": None},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -90,6 +90,9 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
         obj = os.path.join(OBJDIR, f"lu_kernels_{lo}_{hi}.o")
         units.append([HIPCC, *CFLAGS, f"-DLUB_N_LO={lo}", f"-DLUB_N_HI={hi}", "-c",
                       os.path.join(CSRC, "lu_kernels_inst.hip"), "-o", obj])
+    for lo, hi in ((17, 24), (25, 32)):
+        units.append([HIPCC, *CFLAGS, f"-DLUB_RF_LO={lo}", f"-DLUB_RF_HI={hi}", "-c",
+                      os.path.join(CSRC, "lu_refined_inst.hip"), "-o", os.path.join(OBJDIR, f"lu_refined_{lo}_{hi}.o")])
     for piv in (0, 1, 3, 4):  # the serial rule (2) runs the first-max kernel (1)
         units.append([HIPCC, *CFLAGS, f"-DLUB_RAGGED_PIV={piv}", "-c", os.path.join(CSRC, "lu_ragged_inst.hip"),
                       "-o", os.path.join(OBJDIR, f"lu_ragged_{piv}.o")])

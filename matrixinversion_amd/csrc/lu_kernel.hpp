@@ -245,13 +245,17 @@ __device__ __forceinline__ int chunk_lds_offset(int base, int lane, int it) {
   }
 }
 
-template <int N, int L>
+// NT: non-temporal loads (the data is read once); ordinary loads otherwise.
+template <int N, int L, bool NT = true>
 __device__ __forceinline__ void tile_fetch(const float* __restrict__ src, TileRegs<N, L>& r, int lane) {
   using T = TileRegs<N, L>;
 #pragma unroll
   for (int it = 0; it < T::IT; ++it) {
     const int c = lane + kWave * it;
-    if (T::CH % kWave == 0 || c < T::CH) r.v[it] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + c);
+    if (T::CH % kWave == 0 || c < T::CH) {
+      if constexpr (NT) r.v[it] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + c);
+      else r.v[it] = reinterpret_cast<const f32x4*>(src)[c];
+    }
   }
 }
 

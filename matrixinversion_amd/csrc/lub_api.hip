@@ -39,6 +39,8 @@ const lub::KernelTable& table() {
     lub_register_ragged_1(&g_table);
     lub_register_ragged_3(&g_table);
     lub_register_ragged_4(&g_table);
+    lub_register_refined_17_24(&g_table);
+    lub_register_refined_25_32(&g_table);
     g_table.ragged[LUB_PIVOT_SERIAL] = g_table.ragged[LUB_PIVOT_PARALLEL];  // same pivots, same kernel
   });
   return g_table;

@@ -18,6 +18,8 @@ constexpr int kLayoutMaxN[kNumLayout] = {32, 16};
 struct KernelTable {
   const void* plain[kMaxN + 1][kNumPivot][kNumLayout];
   const void* ragged[kNumPivot];
+  // inverse + one refinement step (n = 17..32): [n][0] no pivoting, [n][1] first-max
+  const void* refined[kMaxN + 1][2];
 };
 
 // Per-n launch choices shared by the plain and the ragged paths (set by the
@@ -47,3 +49,5 @@ extern "C" void lub_register_ragged_0(lub::KernelTable* t);
 extern "C" void lub_register_ragged_1(lub::KernelTable* t);
 extern "C" void lub_register_ragged_3(lub::KernelTable* t);
 extern "C" void lub_register_ragged_4(lub::KernelTable* t);
+extern "C" void lub_register_refined_17_24(lub::KernelTable* t);
+extern "C" void lub_register_refined_25_32(lub::KernelTable* t);

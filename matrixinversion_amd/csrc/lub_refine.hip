@@ -19,6 +19,7 @@
 // n = 32 matrix arrives as 16-byte global loads and needs no zero fill.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "lub_internal.h"
@@ -151,11 +152,30 @@ int lu_batched_refine(const float* A, float* X, int n, int64_t batch, void* stre
   return enqueue_refine(A, X, n, batch, static_cast<hipStream_t>(stream));
 }
 
+// The fused kernel (lu_refined.hpp) serves n = 17..32 with no pivoting or
+// the first-max rule in the 16-lane layout; other sizes and rules run the
+// three steps: copy, plain inverse in place, refinement pass.
 int lu_batched_inverse_refined(const float* A, float* X, int* piv, int n, int64_t batch, int flags, void* stream) {
   const int rc = check_args(A, X, n, batch);
   if (rc >= 0) return rc;
   if (A == X) return LUB_EINVAL_ARG;  // A must survive for the residual
+  if ((flags & ~(LUB_PIVOT_MASK | LUB_LAYOUT_MASK)) != 0 || (flags & LUB_PIVOT_MASK) > LUB_PIVOT_GEPP ||
+      (flags & LUB_LAYOUT_MASK) == LUB_LAYOUT_MASK)
+    return LUB_EFLAGS;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const int rule = flags & LUB_PIVOT_MASK;
+  const bool fused = n >= 17 && (rule == LUB_PIVOT_NONE || rule == LUB_PIVOT_PARALLEL || rule == LUB_PIVOT_SERIAL) &&
+                     (flags & LUB_LAYOUT_MASK) != LUB_LAYOUT_QUAD4;
+  if (fused) {
+    const void* fn = lub::kernel_table().refined[n][rule == LUB_PIVOT_NONE ? 0 : 1];
+    if (fn == nullptr) return LUB_ENODEVICE;
+    int64_t ntiles = (batch + 3) / 4;
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ntiles, int64_t(1) << 22));
+    int vec = ((reinterpret_cast<uintptr_t>(A) & 15u) == 0) ? 1 : 0;
+    void* args[] = {const_cast<float**>(&A), &X, &piv, &batch, &ntiles, &vec};
+    if (hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, s) != hipSuccess) return LUB_EHIP;
+    return LUB_OK;
+  }
   if (hipMemcpyAsync(X, A, size_t(batch) * n * n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return LUB_EHIP;
   int r = lu_batched_inverse_async64(X, piv, n, batch, flags, stream);
   if (r != LUB_OK) return r;

@@ -49,23 +49,60 @@ def test_refinement_reduces_error(gpu, k):
         assert np.median(e1) <= np.median(e0)
 
 
-def test_inverse_refined_entry_keeps_a(gpu):
-    n, batch = 32, 40
-    a = workloads.conditioned_matrices(5, batch, n, 3)
-    A = gpu.DeviceBuffer.from_array(a)
+def _inverse_refined(gpu, a, flags, offset=0):
+    """lu_batched_inverse_refined through the C ABI (A at `offset` bytes into
+    its buffer): (refined inverse, pivots, A after the call)."""
+    from matrixinversion_amd._native import check, lib
+
+    batch, n, _ = a.shape
+    A = gpu.DeviceBuffer(a.nbytes + offset)
     X = gpu.DeviceBuffer(a.nbytes)
     P = gpu.DeviceBuffer(batch * n * 4)
-    print("[case] lu_batched_inverse_refined", flush=True)
     try:
-        from matrixinversion_amd._native import check, lib
-        check(lib().lu_batched_inverse_refined(A.ptr, X.ptr, P.ptr, n, batch, 0x1, None), "refined")
+        A.upload(a, offset=offset)
+        check(lib().lu_batched_inverse_refined(A.ptr + offset, X.ptr, P.ptr, n, batch, flags, None), "refined")
         gpu.synchronize()
-        x = X.download(a.shape, np.float32)
-        assert np.array_equal(A.download(a.shape, np.float32), a), "A was modified"
-        _, x1 = gpu.invert_refined(a, 0x1)
-        assert np.array_equal(x.view(np.uint32), x1.view(np.uint32))
-        p = P.download((batch, n), np.int32)
-        assert (p == oracle.batch_pivots(a, oracle.PIV_FIRSTMAX)).all()
+        return (X.download(a.shape, np.float32), P.download((batch, n), np.int32),
+                A.download(a.shape, np.float32, offset=offset))
     finally:
         for b in (A, X, P):
             b.close()
+
+
+def _within_step(a, x0, x1, ulps=4):
+    """x1 equals one refinement step of x0 (FP64 residual) within `ulps`."""
+    ref = _step_reference(a, x0)
+    scale = np.abs(ref).max(axis=(1, 2), keepdims=True)
+    return np.all(np.abs(x1 - ref) <= ulps * 2.0 ** -24 * (np.abs(ref) + 1e-3 * scale))
+
+
+def test_inverse_refined_entry_keeps_a(gpu):
+    n, batch = 32, 40
+    a = workloads.conditioned_matrices(5, batch, n, 3)
+    print("[case] lu_batched_inverse_refined", flush=True)
+    x, p, a_after = _inverse_refined(gpu, a, 0x1)
+    assert np.array_equal(a_after, a), "A was modified"
+    x0, x1 = gpu.invert_refined(a, 0x1)
+    assert _within_step(a, x0, x), "fused refinement differs from one step of the plain inverse"
+    assert np.all(np.abs(x - x1) <= 4 * 2.0 ** -24 * (np.abs(x1) + 1e-3 * np.abs(x1).max())), \
+        "fused and three-step refinement disagree"
+    assert (p == oracle.batch_pivots(a, oracle.PIV_FIRSTMAX)).all()
+
+
+@pytest.mark.parametrize("flags", [0x0, 0x1, 0x2])
+def test_fused_refinement_every_size(gpu, flags):
+    """n = 17..32 take the fused kernel (inverse + refinement with A read
+    once): its result is one refinement step of the plain inverse, within
+    4 ulp, for batches ending in a partial tile and for an unaligned A (the
+    element-wise tile path); pivots equal the oracle's; A is untouched."""
+    rule = oracle.PIV_NONE if flags == 0 else oracle.PIV_FIRSTMAX
+    for n in range(17, 33):
+        for batch, offset in ((23, 0), (9, 4)):
+            a = oracle.dominant_matrices(700 + n, batch, n) if flags == 0 else \
+                workloads.conditioned_matrices(800 + n, batch, n, 2)
+            print(f"[case] fused refine n={n} flags={flags:#x} batch={batch} offset={offset}", flush=True)
+            x, p, a_after = _inverse_refined(gpu, a, flags, offset)
+            assert np.array_equal(a_after, a), f"n={n}: A was modified"
+            x0, _ = gpu.invert(a, flags)
+            assert _within_step(a, x0, x), f"n={n}: not one refinement step of the plain inverse"
+            assert (p == oracle.batch_pivots(a, rule)).all(), f"n={n}: pivots"
