@@ -150,6 +150,16 @@ int lu_ragged_invalid_count(const lub_ragged_plan* plan, int* out);
 int lu_tuning_set(const signed char* layout, const signed char* grab);
 int lu_tuning_get(signed char* layout, signed char* grab);
 
+/* LDS row padding of the plain kernels, per n (33 entries indexed by n):
+ * pad[n] = 1 pads each row of a staged n x n matrix by 4 floats (16 bytes),
+ * 0 leaves rows unpadded; only n % 4 == 0 may be padded (LUB_EINVAL_ARG
+ * otherwise).  Default: padded where n % 4 == 0, except n = 28.  Results
+ * do not depend on it.  Replaces the reference's bank swizzle of the
+ * shared-memory rows (swzl/luBatchedInplace.cuh:12-20), a compile-time
+ * choice there. */
+int lu_tuning_set_lds_pad(const signed char* pad);
+int lu_tuning_get_lds_pad(signed char* pad);
+
 /* ---- LAPACK-style outputs -----------------------------------------------
  * The same inversion with getrf/getri-shaped outputs: ipiv[k] (1-based) is
  * the row swapped with row k at step k, info is 0, or k + 1 for the first

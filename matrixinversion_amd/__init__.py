@@ -9,7 +9,7 @@ serving driver.
 from .api import (
     LAYOUT_AUTO, LAYOUT_QUAD4, LAYOUT_ROW16, MAX_N, PIVOT_NAMES, PIVOT_NONE, PIVOT_PARALLEL, PIVOT_SERIAL,
     PIVOT_TREE, PIVOT_GEPP, RaggedPlan, invert, invert_lapack, invert_ragged, invert_refined, lu_batched_inverse,
-    lu_batched_inverse_ragged, pack_ragged, refine, residual, tuning_get, tuning_set, version, build_id,
+    lu_batched_inverse_ragged, pack_ragged, refine, residual, tuning_get, tuning_set, version, build_id, lds_pad_get, lds_pad_set,
 )
 from .device import DeviceBuffer, Event, Graph, Stream, device_count, device_info, synchronize
 
@@ -17,6 +17,6 @@ __all__ = [
     "LAYOUT_AUTO", "LAYOUT_QUAD4", "LAYOUT_ROW16", "MAX_N", "PIVOT_NAMES", "PIVOT_NONE", "PIVOT_PARALLEL",
     "PIVOT_SERIAL", "PIVOT_TREE", "PIVOT_GEPP", "RaggedPlan", "invert", "invert_lapack", "invert_ragged", "invert_refined",
     "lu_batched_inverse", "lu_batched_inverse_ragged", "pack_ragged", "refine", "residual", "tuning_get",
-    "tuning_set", "version", "build_id", "DeviceBuffer", "Event", "Graph", "Stream", "device_count", "device_info",
+    "tuning_set", "version", "build_id", "lds_pad_get", "lds_pad_set", "DeviceBuffer", "Event", "Graph", "Stream", "device_count", "device_info",
     "synchronize",
 ]

@@ -45,6 +45,8 @@ ABI = {
     "lu_server_counters": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i)]),
     "lu_server_stats": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "lu_tuning_get": (_i, [ctypes.c_char_p, ctypes.c_char_p]),
+    "lu_tuning_set_lds_pad": (_i, [ctypes.c_char_p]),
+    "lu_tuning_get_lds_pad": (_i, [ctypes.c_char_p]),
     "lu_status_string": (ctypes.c_char_p, [_i]),
     "lu_last_error": (ctypes.c_char_p, []),
     "lu_version": (ctypes.c_char_p, []),

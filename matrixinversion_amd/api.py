@@ -266,6 +266,21 @@ def tuning_set(layout=None, grab=None) -> None:
     check(lib().lu_tuning_set(lay.tobytes(), grb.tobytes()), "lu_tuning_set")
 
 
+def lds_pad_get():
+    """Per-n LDS row padding of the plain kernels (33 entries, 1 = padded)."""
+    pad = ctypes.create_string_buffer(33)
+    check(lib().lu_tuning_get_lds_pad(pad), "lu_tuning_get_lds_pad")
+    return np.frombuffer(pad.raw, dtype=np.int8).copy()
+
+
+def lds_pad_set(pad) -> None:
+    """Replace the per-n LDS row padding (only n % 4 == 0 may be 1)."""
+    p = np.asarray(pad, dtype=np.int8)
+    if p.shape != (33,):
+        raise ValueError("the table holds 33 entries (index n, entry 0 unused)")
+    check(lib().lu_tuning_set_lds_pad(p.tobytes()), "lu_tuning_set_lds_pad")
+
+
 def version() -> str:
     return lib().lu_version().decode()
 

@@ -8,6 +8,7 @@ n, read by liblubatch whenever flags carry LUB_LAYOUT_AUTO:
   layout[n]  0 = 16 lanes per matrix (4 matrices per wave), 1 = one 4-lane
              quad per matrix (16 per wave; plain path n <= 16, ragged n <= 12)
   grab[n]    ragged path: wave tiles per work item (up to 64 matrices)
+  pad[n]     plain path, n % 4 == 0: LDS rows padded by 4 floats (1) or not (0)
 
 This module measures both on the current device and writes the table, which
 load_table() installs in a process (the library's built-in defaults are a
@@ -86,6 +87,38 @@ def tune_layout(n: int, batch: int, stream: Stream, reps: int = 20) -> tuple[int
     return (1 if best == "quad4" else 0), times
 
 
+def tune_pad(n: int, layout: int, batch: int, stream: Stream, reps: int = 20) -> tuple[int, dict]:
+    """Plain path, n % 4 == 0: LDS rows padded by 4 floats or not, in the
+    chosen layout."""
+    a = uniform_matrices(1500 + n, batch, n)
+    src = DeviceBuffer.from_array(a)
+    work = DeviceBuffer(a.nbytes)
+    piv = DeviceBuffer(batch * n * 4)
+    flag = api.LAYOUT_QUAD4 if layout == 1 else api.LAYOUT_ROW16
+    base = api.lds_pad_get()
+    times = {}
+    try:
+        for pad in (0, 1):
+            p = base.copy()
+            p[n] = pad
+            api.lds_pad_set(p)
+
+            def run():
+                copy_kernel(work, src, batch * n * n, stream)
+                api.lu_batched_inverse(work, n, batch, api.PIVOT_PARALLEL | flag, piv, stream)
+
+            def copy_only():
+                copy_kernel(work, src, batch * n * n, stream)
+
+            times[pad] = _time(run, stream, reps) - _time(copy_only, stream, reps)
+    finally:
+        api.lds_pad_set(base)
+        for b in (src, work, piv):
+            b.close()
+    best = min(times, key=times.get)
+    return best, times
+
+
 def tune_grab(n: int, layout: int, batch: int, stream: Stream, reps: int = 20) -> tuple[int, dict]:
     """Ragged path: wave tiles per work item for one n under `layout`."""
     mpw = 16 if (layout == 1 and n <= RAGGED_QUAD_MAX_N) else 4
@@ -131,6 +164,7 @@ def autotune(sizes=range(1, 33), plain_batch: int = 262144, ragged_batch: int = 
     layout, grab = api.tuning_get()
     layout = layout.copy()
     grab = grab.copy()
+    pad = api.lds_pad_get()
     detail = {}
     for n in sizes:
         t0 = time.perf_counter()
@@ -139,17 +173,23 @@ def autotune(sizes=range(1, 33), plain_batch: int = 262144, ragged_batch: int = 
         else:
             layout[n], lt = 0, {}
         api.tuning_set(layout, grab)
+        if n % 4 == 0:
+            pad[n], pt = tune_pad(n, int(layout[n]), plain_batch, stream)
+            api.lds_pad_set(pad)
+        else:
+            pt = {}
         grab[n], gt = tune_grab(n, int(layout[n]), ragged_batch, stream)
         api.tuning_set(layout, grab)
-        detail[n] = {"layout_ms": lt, "grab_ms": {str(k): v for k, v in gt.items()}}
+        detail[n] = {"layout_ms": lt, "pad_ms": {str(k): v for k, v in pt.items()},
+                     "grab_ms": {str(k): v for k, v in gt.items()}}
         if log:
-            log(f"[tune] n={n:2d} layout={'quad4' if layout[n] else 'row16'} grab={grab[n]} "
-                f"({time.perf_counter() - t0:.1f}s) {lt} {gt}")
+            log(f"[tune] n={n:2d} layout={'quad4' if layout[n] else 'row16'} pad={pad[n]} grab={grab[n]} "
+                f"({time.perf_counter() - t0:.1f}s) {lt} {pt} {gt}")
     stream.close()
     info = device_info(0)
     return {"device": info.name, "arch": info.arch, "layout": [int(x) for x in layout],
-            "grab": [int(x) for x in grab], "plain_batch": plain_batch, "ragged_batch": ragged_batch,
-            "detail": detail}
+            "grab": [int(x) for x in grab], "pad": [int(x) for x in pad], "plain_batch": plain_batch,
+            "ragged_batch": ragged_batch, "detail": detail}
 
 
 def tune_mix(batch: int = 500_000, n_lo: int = 2, n_hi: int = 32, rounds: int = 1, reps: int = 10,
@@ -171,6 +211,7 @@ def tune_mix(batch: int = 500_000, n_lo: int = 2, n_hi: int = 32, rounds: int = 
     P = DeviceBuffer(int(ns.sum()) * 4)
     plan = api.RaggedPlan(batch)
     layout, grab = (x.copy() for x in api.tuning_get())
+    pad = api.lds_pad_get()
 
     def mix_ms(lay, grb) -> float:
         api.tuning_set(lay, grb)
@@ -209,13 +250,16 @@ def tune_mix(batch: int = 500_000, n_lo: int = 2, n_hi: int = 32, rounds: int = 
         for b in (src, work, O, NS, PO, P):
             b.close()
         stream.close()
-    return {"layout": [int(x) for x in layout], "grab": [int(x) for x in grab], "mix_batch": batch,
+    return {"layout": [int(x) for x in layout], "grab": [int(x) for x in grab], "pad": [int(x) for x in pad],
+            "mix_batch": batch,
             "mix_ms_start": start, "mix_ms_best": best, "mix_ms_final": final, "trials": trials}
 
 
 def apply_table(table: dict) -> None:
     """Install a table written by autotune() (33 entries per column)."""
     api.tuning_set(np.array(table["layout"], np.int8), np.array(table["grab"], np.int8))
+    if "pad" in table:
+        api.lds_pad_set(np.array(table["pad"], np.int8))
 
 
 def load_table(path: str) -> None:
@@ -233,12 +277,13 @@ def main(argv=None) -> int:
                     help="class: each n timed alone (plain layout, ragged grab); mix: the ragged table "
                          "tuned on the mixed batch (--mix-batch matrices, n ~ U{2..32})")
     ap.add_argument("--mix-batch", type=int, default=500_000)
+    ap.add_argument("--rounds", type=int, default=1, help="mix: coordinate-descent sweeps over the classes")
     args = ap.parse_args(argv)
     lo, _, hi = args.sizes.partition("-")
     sizes = range(int(lo), int(hi or lo) + 1)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     if args.objective == "mix":
-        table = tune_mix(args.mix_batch, log=log)
+        table = tune_mix(args.mix_batch, rounds=args.rounds, log=log)
     else:
         table = autotune(sizes, args.plain_batch, args.ragged_batch, log=log)
     synchronize()

@@ -60,27 +60,38 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 enum Pivot : int { kPivNone = 0, kPivParallel = 1, kPivSerial = 2, kPivTree = 3, kPivGepp = 4 };
 
 // Tile geometry for n = N with L lanes per matrix (all sizes in floats).
-// Rows whose length is a multiple of 8 floats get 4 floats of padding so that
-// 8 consecutive rows start in 8 different 16-byte bank groups; for n % 4 == 0
-// consecutive matrices start L banks apart so the lane groups of one
-// half-wave column access hit disjoint banks.  For n % 4 != 0 a matrix is
-// stored contiguously (row stride n): in the plain layout (V = 0) the whole
-// tile is contiguous, as in HBM, so the tile's dwordx4 chunks (which cross
-// row and matrix boundaries) stage unchanged; in the gathered layout (V = 1,
-// the ragged kernel, whose matrices arrive one by one) each matrix starts on
-// a 16-byte boundary so that its own chunks stage as aligned dwordx4.
+// V is a bit set: bit 0 selects the gathered layout (below), bit 1 flips the
+// row padding from its default.
+// Row padding (n % 4 == 0 only): a padded row is n + 4 floats long, so that
+// consecutive rows start in different 16-byte bank groups.  By default rows
+// are padded when n % 8 == 0 (otherwise 8 consecutive rows would start in
+// the same bank group); the tuning table can flip that per n
+// (lu_tuning_set_lds_pad), which changes the LDS addresses only.
+// For n % 4 == 0 consecutive matrices start L banks apart so the lane groups
+// of one half-wave column access hit disjoint banks.  For n % 4 != 0 a
+// matrix is stored contiguously (row stride n): in the plain layout the
+// whole tile is contiguous, as in HBM, so the tile's dwordx4 chunks (which
+// cross row and matrix boundaries) stage unchanged; in the gathered layout
+// (V & 1, the ragged kernel, whose matrices arrive one by one) each matrix
+// starts on a 16-byte boundary so that its own chunks stage as aligned
+// dwordx4.
 template <int N, int L, int V = 0>
 struct Geo {
   static_assert(L == 16 || L == 4, "a matrix is owned by a DPP row or a quad");
   static_assert(N >= 1 && N <= (L == 16 ? 32 : 16), "n too large for this lane grouping");
+  static constexpr bool GATHER = (V & 1) != 0;
+  static constexpr bool PAD = (N % 4 == 0) && ((N % 8 == 0) != ((V & 2) != 0));
   static constexpr int MPW = kWave / L;          // matrices per wave (tile)
   static constexpr int R = (N + L - 1) / L;      // rows (columns) per lane
-  static constexpr int LD = (N % 8 == 0) ? N + 4 : N;
-  static constexpr int MS = (N % 4 == 0) ? ((N * LD + 31) / 32) * 32 + L : (V == 0 ? N * N : (N * N + 3) / 4 * 4);
+  static constexpr int LD = PAD ? N + 4 : N;
+  static constexpr int MS = (N % 4 == 0) ? ((N * LD + 31) / 32) * 32 + L : (GATHER ? (N * N + 3) / 4 * 4 : N * N);
   static constexpr int TILE = MPW * MS;
   static constexpr int PERM = TILE;               // int32[MPW][32] after the tile
   static constexpr int LDS_FLOATS = TILE + MPW * 32;
 };
+
+// The padding variant that flips the default (exists for n % 4 == 0).
+constexpr int kFlipPad = 2;
 
 // Per-lane register blocks of a matrix: R columns (or rows) of n floats, and
 // one int / float per column.
@@ -217,9 +228,9 @@ struct TileRegs {
 template <int N>
 constexpr bool kChunkSplits = (N % 4 == 0) && (256 % N == 0) && ((N * N) % 256 == 0 || 256 % (N * N) == 0);
 
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ int chunk_lds_base(int lane) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   constexpr int NN = N * N;
   if constexpr (kChunkSplits<N>) {
     const int f = 4 * lane;  // within the first iteration
@@ -230,9 +241,9 @@ __device__ __forceinline__ int chunk_lds_base(int lane) {
   }
 }
 
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ int chunk_lds_offset(int base, int lane, int it) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   constexpr int NN = N * N;
   if constexpr (kChunkSplits<N>) {
     const int f0 = 256 * it;  // iteration it adds 256 floats: whole rows / matrices
@@ -259,21 +270,21 @@ __device__ __forceinline__ void tile_fetch(const float* __restrict__ src, TileRe
   }
 }
 
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ void tile_stage(const TileRegs<N, L>& r, float* lds, int lane) {
   using T = TileRegs<N, L>;
-  const int base = chunk_lds_base<N, L>(lane);
+  const int base = chunk_lds_base<N, L, V>(lane);
 #pragma unroll
   for (int it = 0; it < T::IT; ++it) {
     const int c = lane + kWave * it;
     if (T::CH % kWave == 0 || c < T::CH)
-      *reinterpret_cast<f32x4*>(lds + chunk_lds_offset<N, L>(base, lane, it)) = r.v[it];
+      *reinterpret_cast<f32x4*>(lds + chunk_lds_offset<N, L, V>(base, lane, it)) = r.v[it];
   }
 }
 
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ void tile_load_elementwise(const float* __restrict__ src, float* lds, int valid, int lane) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   constexpr int NN = N * N;
   for (int f = lane; f < G::MPW * NN; f += kWave) {
     const int m = f / NN, e = f - m * NN, i = e / N, j = e - i * N;
@@ -282,18 +293,18 @@ __device__ __forceinline__ void tile_load_elementwise(const float* __restrict__ 
 }
 
 // LDS -> HBM, the mirror of the load; only the `valid` leading matrices are stored.
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ void tile_store(float* __restrict__ dst, const float* lds, int valid, bool vec, int lane) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   using T = TileRegs<N, L>;
   constexpr int NN = N * N;
   if (vec && valid == G::MPW) {
-    const int base = chunk_lds_base<N, L>(lane);
+    const int base = chunk_lds_base<N, L, V>(lane);
 #pragma unroll
     for (int it = 0; it < T::IT; ++it) {
       const int c = lane + kWave * it;
       if (T::CH % kWave == 0 || c < T::CH) {
-        f32x4 v = *reinterpret_cast<const f32x4*>(lds + chunk_lds_offset<N, L>(base, lane, it));
+        f32x4 v = *reinterpret_cast<const f32x4*>(lds + chunk_lds_offset<N, L, V>(base, lane, it));
         __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(dst) + c);
       }
     }
@@ -673,14 +684,14 @@ __device__ __forceinline__ void tile_solve(ColBlock<N, L>& w, const LaneInts<N, 
 
 // Phase 8 for a contiguous tile whose inverses are in LDS: the inverses, the
 // info words and the pivot vectors (prm[h] = perm[l + L*h]) to HBM.
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ void tile_finish(const LaneInts<N, L>& prm, int failure, float* __restrict__ A,
                                             int* __restrict__ piv, int* __restrict__ info, int64_t m0, int valid,
                                             bool vec, float* lds, int lane) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   const int l = lane % L;
   const int g = lane / L;
-  tile_store<N, L>(A + m0 * (N * N), lds, valid, vec, lane);
+  tile_store<N, L, V>(A + m0 * (N * N), lds, valid, vec, lane);
   if (info != nullptr && l == 0 && g < valid) info[m0 + g] = failure < N ? failure + 1 : 0;
   if (piv != nullptr && g < valid) {
 #pragma unroll
@@ -691,13 +702,13 @@ __device__ __forceinline__ void tile_finish(const LaneInts<N, L>& prm, int failu
 }
 
 // Phases 6-8 of the reference's pivot rules for a contiguous tile.
-template <int N, int L>
+template <int N, int L, int V = 0>
 __device__ __forceinline__ void tile_back(ColBlock<N, L>& w, const LaneInts<N, L>& prm, float* __restrict__ A,
                                           int* __restrict__ piv, int* __restrict__ info, int64_t m0, int valid,
                                           bool vec, float* lds, int lane) {
   int failure = N;
-  tile_solve<N, L>(w, prm, lds, lane, info != nullptr ? &failure : nullptr);
-  tile_finish<N, L>(prm, failure, A, piv, info, m0, valid, vec, lds, lane);
+  tile_solve<N, L, V>(w, prm, lds, lane, info != nullptr ? &failure : nullptr);
+  tile_finish<N, L, V>(prm, failure, A, piv, info, m0, valid, vec, lds, lane);
 }
 
 // --------------------------------------------- partial pivoting (GEPP)
@@ -844,20 +855,20 @@ __device__ __forceinline__ void tile_gepp(LaneInts<N, L>& prm, float* lds, int l
 // (checked per instantiation with -Rpass-analysis=kernel-resource-usage:
 // n = 25..31 with the first-max rules, 25..30 with the tree rule); elsewhere
 // 2, the LDS or the register tile being the limit anyway.
-template <int N, int L, int PIV>
+template <int N, int L, int PIV, int V = 0>
 constexpr int kWavesPerEu =
     (L == 16 && N >= 25 && N <= 31 && PIV != kPivGepp && !(PIV == kPivTree && N == 31) &&
-     160 * 1024 / (Geo<N, L>::LDS_FLOATS * 4) > 8) ? 3 : 2;
+     160 * 1024 / (Geo<N, L, V>::LDS_FLOATS * 4) > 8) ? 3 : 2;
 
 // Each workgroup (one wave) strides over tiles.  The next full tile is
 // fetched into registers once the current one has left LDS for registers
 // (after the column read), so with several tiles per workgroup its HBM reads
 // overlap this tile's Gauss-Jordan, the longest phase.
 // amdgpu_waves_per_eu: see kWavesPerEu.
-template <int N, int L, int PIV>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N, L, PIV>))) void lu_inverse_kernel(
+template <int N, int L, int PIV, int V = 0>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N, L, PIV, V>))) void lu_inverse_kernel(
     float* __restrict__ A, int* __restrict__ piv, int* __restrict__ info, int64_t batch, int64_t ntiles, int vec) {
-  using G = Geo<N, L>;
+  using G = Geo<N, L, V>;
   __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
   const int lane = threadIdx.x;
   const int64_t stride = gridDim.x;
@@ -870,8 +881,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPer
     const int64_t m0 = t * G::MPW;
     const int64_t left = batch - m0;
     const int valid = left < G::MPW ? (int)left : G::MPW;
-    if (have) tile_stage<N, L>(next, lds, lane);
-    else tile_load_elementwise<N, L>(A + m0 * (N * N), lds, valid, lane);
+    if (have) tile_stage<N, L, V>(next, lds, lane);
+    else tile_load_elementwise<N, L, V>(A + m0 * (N * N), lds, valid, lane);
     wave_sync();
     int prm[G::R];
     if constexpr (PIV == kPivGepp) {
@@ -879,15 +890,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPer
       have = tn < ntiles && full(tn);
       if (have) tile_fetch<N, L>(A + tn * G::MPW * (N * N), next, lane);
       int failure = N;
-      tile_gepp<N, L>(prm, lds, lane, info != nullptr ? &failure : nullptr);
-      tile_finish<N, L>(prm, failure, A, piv, info, m0, valid, vec != 0, lds, lane);
+      tile_gepp<N, L, V>(prm, lds, lane, info != nullptr ? &failure : nullptr);
+      tile_finish<N, L, V>(prm, failure, A, piv, info, m0, valid, vec != 0, lds, lane);
     } else {
       float w[G::R][N];
-      tile_front<N, L, PIV>(w, prm, lds, lane);
+      tile_front<N, L, PIV, V>(w, prm, lds, lane);
       const int64_t tn = t + stride;
       have = tn < ntiles && full(tn);
       if (have) tile_fetch<N, L>(A + tn * G::MPW * (N * N), next, lane);
-      tile_back<N, L>(w, prm, A, piv, info, m0, valid, vec != 0, lds, lane);
+      tile_back<N, L, V>(w, prm, A, piv, info, m0, valid, vec != 0, lds, lane);
     }
   }
 }

@@ -16,17 +16,24 @@
 
 namespace {
 
-template <int N, int L>
-void register_layout(lub::KernelTable* t, int layout) {
-  t->plain[N][lub::kPivNone][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivNone>);
-  t->plain[N][lub::kPivParallel][layout] =
-      reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivParallel>);
+template <int N, int L, int V>
+void register_variant(const void* (&slot)[lub::kMaxN + 1][lub::kNumPivot][lub::kNumLayout], int layout) {
+  slot[N][lub::kPivNone][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivNone, V>);
+  slot[N][lub::kPivParallel][layout] =
+      reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivParallel, V>);
   // The serial rule's pivots are the first-max rule's: the same kernel (DPP
   // argmax, with the lane-serial scan on ties and NaN) serves both flags.
-  t->plain[N][lub::kPivSerial][layout] =
-      reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivParallel>);
-  t->plain[N][lub::kPivTree][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivTree>);
-  t->plain[N][lub::kPivGepp][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivGepp>);
+  slot[N][lub::kPivSerial][layout] =
+      reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivParallel, V>);
+  slot[N][lub::kPivTree][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivTree, V>);
+  slot[N][lub::kPivGepp][layout] = reinterpret_cast<const void*>(&lub::lu_inverse_kernel<N, L, lub::kPivGepp, V>);
+}
+
+// The default row padding, and for n % 4 == 0 the flipped one.
+template <int N, int L>
+void register_layout(lub::KernelTable* t, int layout) {
+  register_variant<N, L, 0>(t->plain, layout);
+  if constexpr (N % 4 == 0) register_variant<N, L, lub::kFlipPad>(t->plain_flip, layout);
 }
 
 template <int N>

@@ -48,16 +48,20 @@ const lub::KernelTable& table() {
 
 // Per-n launch choices.  Defaults from MI355X measurements at 1M matrices
 // (quads for n <= 12; one wave tile per ragged queue descriptor), replaced
-// as a whole by lu_tuning_set (the autotuner's table).
+// by lu_tuning_set (layout, grab) and lu_tuning_set_lds_pad (pad): the
+// autotuner's table.
 std::mutex g_tuning_mutex;
 lub::Tuning g_tuning = [] {
   lub::Tuning t;
   for (int n = 0; n <= lub::kMaxN; ++n) {
     // measured on MI355X (python -m matrixinversion_amd.autotune): quads win
     // up to n = 12; ragged work items of 8 row tiles (4 at n = 32) beat
-    // single tiles by 5-10 % for n >= 13
+    // single tiles by 5-10 % for n >= 13; tuned on the mixed batch
+    // (--objective mix) no other choice wins by more than 0.5 %, n = 30
+    // taking 16 row tiles; LDS rows padded for n % 4 == 0 except n = 28
     t.layout[n] = n <= 12 ? 1 : 0;
-    t.grab[n] = n <= 12 ? 1 : (n == 32 ? 4 : 8);
+    t.grab[n] = n <= 12 ? 1 : (n == 32 ? 4 : (n == 30 ? 16 : 8));
+    t.pad[n] = lub::initial_pad(n);
   }
   return t;
 }();
@@ -161,7 +165,12 @@ int enqueue_plain(float* A, int* piv, int* info, int n, int64_t batch, int flags
   int rc = validate(A, piv, n, batch, flags);
   if (rc != LUB_OK || batch == 0) return rc;
   const int layout = choose_layout(n, flags);
-  const void* fn = table().plain[n][flags & LUB_PIVOT_MASK][layout];
+  bool flip = false;
+  if (n % 4 == 0) {
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    flip = g_tuning.pad[n] != lub::geo_pad(n);
+  }
+  const void* fn = (flip ? table().plain_flip : table().plain)[n][flags & LUB_PIVOT_MASK][layout];
   if (fn == nullptr) return LUB_ENODEVICE;
   const int per_tile = 64 / lub::kLayoutLanes[layout];  // matrices per wave tile
   int64_t ntiles = (batch + per_tile - 1) / per_tile;
@@ -189,14 +198,29 @@ extern "C" {
 
 int lu_tuning_set(const signed char* layout, const signed char* grab) {
   if (layout == nullptr || grab == nullptr) return LUB_ENULL;
-  lub::Tuning t;
-  for (int n = 0; n <= lub::kMaxN; ++n) {
+  for (int n = 0; n <= lub::kMaxN; ++n)
     if (layout[n] < 0 || layout[n] > 1 || grab[n] < 1) return LUB_EINVAL_ARG;
-    t.layout[n] = static_cast<int8_t>(layout[n]);
-    t.grab[n] = static_cast<int8_t>(grab[n]);
-  }
   std::lock_guard<std::mutex> lock(g_tuning_mutex);
-  g_tuning = t;
+  for (int n = 0; n <= lub::kMaxN; ++n) {
+    g_tuning.layout[n] = static_cast<int8_t>(layout[n]);
+    g_tuning.grab[n] = static_cast<int8_t>(grab[n]);
+  }
+  return LUB_OK;
+}
+
+int lu_tuning_set_lds_pad(const signed char* pad) {
+  if (pad == nullptr) return LUB_ENULL;
+  for (int n = 0; n <= lub::kMaxN; ++n)
+    if (pad[n] < 0 || pad[n] > 1 || (pad[n] == 1 && (n % 4 != 0 || n == 0))) return LUB_EINVAL_ARG;
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  for (int n = 0; n <= lub::kMaxN; ++n) g_tuning.pad[n] = static_cast<int8_t>(pad[n]);
+  return LUB_OK;
+}
+
+int lu_tuning_get_lds_pad(signed char* pad) {
+  if (pad == nullptr) return LUB_ENULL;
+  lub::Tuning t = lub::current_tuning();
+  for (int n = 0; n <= lub::kMaxN; ++n) pad[n] = t.pad[n];
   return LUB_OK;
 }
 

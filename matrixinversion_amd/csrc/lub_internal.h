@@ -17,6 +17,8 @@ constexpr int kLayoutMaxN[kNumLayout] = {32, 16};
 // its range of N and registers the host stubs here).
 struct KernelTable {
   const void* plain[kMaxN + 1][kNumPivot][kNumLayout];
+  // n % 4 == 0: the same kernels with the row padding flipped from its default
+  const void* plain_flip[kMaxN + 1][kNumPivot][kNumLayout];
   const void* ragged[kNumPivot];
   // inverse + one refinement step (n = 17..32): [n][0] no pivoting, [n][1] first-max
   const void* refined[kMaxN + 1][2];
@@ -27,7 +29,16 @@ struct KernelTable {
 struct Tuning {
   int8_t layout[kMaxN + 1];  // 0: 16-lane rows, 1: quads
   int8_t grab[kMaxN + 1];    // ragged: wave tiles per work item
+  int8_t pad[kMaxN + 1];     // plain path, n % 4 == 0: LDS rows padded by 4 floats (1) or not (0)
 };
+
+// LDS row padding compiled into a kernel without the flip bit
+// (Geo<N, L, V>::PAD for V & 2 == 0): rows of n % 8 == 0.
+constexpr int8_t geo_pad(int n) { return (n > 0 && n % 8 == 0) ? 1 : 0; }
+// Initial padding table of the plain path: every n % 4 == 0 but n = 28
+// (measured on MI355X with the autotuner: padding n = 12 and n = 20 saves
+// 4 % and 9 % of the kernel time, at n = 28 it costs 7 %).
+constexpr int8_t initial_pad(int n) { return (n > 0 && n % 4 == 0 && n != 28) ? 1 : 0; }
 Tuning current_tuning();
 const KernelTable& kernel_table();
 

@@ -115,6 +115,25 @@ def test_ragged_and_tuning_validation_codes():
         api.tuning_set(layout, grab)
     assert (api.tuning_get()[1] == grab).all()
 
+    pad = api.lds_pad_get()
+    assert pad.shape == (33,) and pad.tolist() == [1 if (n % 4 == 0 and n not in (0, 28)) else 0 for n in range(33)]
+    bad = pad.copy()
+    bad[5] = 1                                                         # only n % 4 == 0 can be padded
+    assert lib.lu_tuning_set_lds_pad(bad.tobytes()) == 8
+    bad = pad.copy()
+    bad[8] = 2
+    assert lib.lu_tuning_set_lds_pad(bad.tobytes()) == 8
+    assert lib.lu_tuning_set_lds_pad(None) == 3 and lib.lu_tuning_get_lds_pad(None) == 3
+    try:
+        p2 = pad.copy()
+        p2[12] = 1
+        p2[32] = 0
+        api.lds_pad_set(p2)
+        assert api.lds_pad_get()[12] == 1 and api.lds_pad_get()[32] == 0
+        assert (api.tuning_get()[1] == grab).all()                     # layout/grab untouched
+    finally:
+        api.lds_pad_set(pad)
+
 
 def test_refine_validation_codes():
     from matrixinversion_amd import _native

@@ -60,12 +60,15 @@ def test_autotuner_installs_a_table_with_identical_results(gpu, tmp_path):
 
     print("[case] autotune small", flush=True)
     layout0, grab0 = gpu.tuning_get()
+    pad0 = gpu.lds_pad_get()
     path = str(tmp_path / "table.json")
     try:
         table = autotune.autotune(sizes=[4, 12, 20, 32], plain_batch=20_000, ragged_batch=20_000)
         with open(path, "w") as f:
             json.dump(table, f)
-        assert len(table["layout"]) == 33 and len(table["grab"]) == 33
+        assert len(table["layout"]) == 33 and len(table["grab"]) == 33 and len(table["pad"]) == 33
+        assert all(table["pad"][n] == 0 for n in range(33) if n % 4 != 0)
+        assert all(set(table["detail"][n]["pad_ms"]) == {"0", "1"} for n in (4, 12, 20, 32))
         a = {n: oracle.uniform_matrices(70 + n, 300, n) for n in (4, 12, 20, 32)}
         before = {n: gpu.invert(m, 0x1) for n, m in a.items()}
         mats = [m for n in sorted(a) for m in a[n][:40]]
@@ -80,3 +83,27 @@ def test_autotuner_installs_a_table_with_identical_results(gpu, tmp_path):
             assert np.array_equal(x0.view(np.uint32), x1.view(np.uint32))
     finally:
         gpu.tuning_set(layout0, grab0)
+        gpu.lds_pad_set(pad0)
+
+
+@pytest.mark.parametrize("flags", [0x1, 0x0, 0x4, 0x3])
+def test_lds_padding_does_not_change_results(gpu, flags):
+    """Every n % 4 == 0 kernel with its LDS row padding flipped from the
+    default gives the default's inverses and pivots bit for bit, in both
+    lane layouts; partial last tiles included."""
+    pad0 = gpu.lds_pad_get()
+    try:
+        for n in (4, 8, 12, 16, 20, 24, 28, 32):
+            a = (oracle.dominant_matrices if flags == 0 else oracle.uniform_matrices)(900 + n, 45, n)
+            for lf in ((0x10, 0x20) if n <= 16 else (0x10,)):
+                print(f"[case] lds pad n={n} flags={flags | lf:#x}", flush=True)
+                gpu.lds_pad_set(pad0)
+                x0, p0 = gpu.invert(a, flags | lf)
+                p2 = pad0.copy()
+                p2[n] = 1 - pad0[n]
+                gpu.lds_pad_set(p2)
+                x1, p1 = gpu.invert(a, flags | lf)
+                assert np.array_equal(x0.view(np.uint32), x1.view(np.uint32)), f"n={n} layout={lf:#x}"
+                assert (p0 == p1).all()
+    finally:
+        gpu.lds_pad_set(pad0)
