@@ -141,13 +141,16 @@ def cpu_baseline_ragged(ns_all: np.ndarray, flat: np.ndarray, offsets: np.ndarra
                       f"{total:.1f} s"}
 
 
-def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 2, n_hi: int = 32) -> dict:
+def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 2, n_hi: int = 32,
+                   graph: bool = True) -> dict:
     """Mixed-n batch (`batch` matrices, n ~ U{n_lo..n_hi}, packed with int64
     offsets): the ragged entry (plan + inversion kernel) eager and replayed
     from a hipGraph, against one plain launch per n (a) over the same matrices
     pre-grouped into per-n contiguous buffers (the plain path's best case: no
     gather cost counted) and (b) on the packed batch itself (gather each class,
-    invert, scatter back).  Returns the measurements; `ok` is False when any
+    invert, scatter back).  graph=False launches everything directly (no
+    hipGraph), for profilers that do not follow graph launches; the step time
+    is then the eager one.  Returns the measurements; `ok` is False when any
     result differs bitwise."""
     import matrixinversion_amd as mia
     from matrixinversion_amd import workloads
@@ -227,12 +230,14 @@ def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 
     bad = int(plan.invalid_count())
 
     # graph-replayed ragged: one graph per resident copy
-    graphs = graphs_of(ragged)
-    restore()
-    settle()
-    t_graph = timed(lambda i: graphs[i].launch())
-    x_graph = bufs[-1].download(flat.shape, np.float32)
-    graph_same = bool(np.array_equal(x_graph.view(np.uint32), x_ragged.view(np.uint32)))
+    graphs, t_graph, graph_same = [], None, True
+    if graph:
+        graphs = graphs_of(ragged)
+        restore()
+        settle()
+        t_graph = timed(lambda i: graphs[i].launch())
+        x_graph = bufs[-1].download(flat.shape, np.float32)
+        graph_same = bool(np.array_equal(x_graph.view(np.uint32), x_ragged.view(np.uint32)))
 
     # baseline: one plain launch per n on per-n contiguous copies of the same matrices
     classes = []
@@ -276,15 +281,20 @@ def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 
     restore_plain()
     settle()
     t_plain = timed(plain)
-    plain_graphs = graphs_of(plain)
-    restore_plain()
-    settle()
-    t_plain_graph = timed(lambda i: plain_graphs[i].launch())
-
-    packed_graphs = graphs_of(plain_packed)
-    restore()
-    settle()
-    t_packed_graph = timed(lambda i: packed_graphs[i].launch())
+    plain_graphs, packed_graphs, t_plain_graph = [], [], None
+    if graph:
+        plain_graphs = graphs_of(plain)
+        restore_plain()
+        settle()
+        t_plain_graph = timed(lambda i: plain_graphs[i].launch())
+        packed_graphs = graphs_of(plain_packed)
+        restore()
+        settle()
+        t_packed_graph = timed(lambda i: packed_graphs[i].launch())
+    else:
+        restore()
+        settle()
+        t_packed_graph = timed(plain_packed)
     x_packed = bufs[-1].download(x_ragged.shape, np.float32)
     packed_same = bool(np.array_equal(x_packed.view(np.uint32), x_ragged.view(np.uint32)))
 
@@ -296,8 +306,11 @@ def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 
         same &= bool(np.array_equal(x_ragged[idx].view(np.uint32), xp.view(np.uint32)))
         pp = pv.download((sel.size, n), np.int32)
         same &= bool(np.array_equal(p_ragged[(poffs[sel][:, None] + np.arange(n)[None, :])], pp))
-    log(f"[ragged] eager {t_eager:.3f} ms, graph {t_graph:.3f} ms, per-n plain {t_plain:.3f} ms "
-        f"(graph {t_plain_graph:.3f} ms; on the packed batch with gather/scatter {t_packed_graph:.3f} ms); "
+    def ms(t):
+        return "-" if t is None else f"{t:.3f}"
+
+    log(f"[ragged] eager {ms(t_eager)} ms, graph {ms(t_graph)} ms, per-n plain {ms(t_plain)} ms "
+        f"(graph {ms(t_plain_graph)} ms; on the packed batch with gather/scatter {ms(t_packed_graph)} ms); "
         f"ragged == plain bitwise: {same}; graph == eager: {graph_same}; invalid n: {bad}")
 
     for g in graphs + plain_graphs + packed_graphs:
@@ -310,33 +323,35 @@ def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 
             b.close()
     stream.close()
     algo = workloads.ragged_bytes(ns, pivots=flags != 0)
-    return {"batch": batch, "graph_ms_mean": t_graph, "eager_ms_mean": t_eager,
+    t_step = t_graph if graph else t_eager
+    return {"batch": batch, "timing": "graph" if graph else "eager", "step_ms_mean": t_step,
+            "graph_ms_mean": t_graph, "eager_ms_mean": t_eager,
             "plain_per_n_ms_mean": t_plain, "plain_per_n_graph_ms_mean": t_plain_graph,
-            "speedup_vs_plain_per_n": t_plain / t_graph,
-            "speedup_vs_plain_per_n_graph": t_plain_graph / t_graph,
+            "speedup_vs_plain_per_n": t_plain / t_step,
+            "speedup_vs_plain_per_n_graph": None if t_plain_graph is None else t_plain_graph / t_step,
             "plain_per_n_packed_graph_ms_mean": t_packed_graph,
-            "speedup_vs_plain_per_n_on_packed_data": t_packed_graph / t_graph,
+            "speedup_vs_plain_per_n_on_packed_data": t_packed_graph / t_step,
             "plain_on_packed_equals_ragged_bitwise": packed_same,
             "ragged_equals_plain_bitwise": same, "graph_equals_eager_bitwise": graph_same,
             "invalid_n": bad, "algorithmic_bytes_per_launch": algo,
-            "pct_hbm": 100 * algo / (t_graph * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+            "pct_hbm": 100 * algo / (t_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
             "plain_launches_per_step": len(classes),
             "ok": same and graph_same and packed_same and bad == 0,
             "baselines": "plain_per_n: one lu_batched_inverse per n on matrices pre-grouped by n outside the timed "
                          "region; plain_per_n_packed: per n, gather the class from the packed batch, invert, "
-                         "scatter back (graph-captured)"}
+                         "scatter back (graph-captured unless timing is eager)"}
 
 
 def bench_ragged(args) -> int:
     """The ragged workload as its own bench line (value = matrices/s of the
-    graph-replayed ragged launch)."""
+    graph-replayed ragged launch; of the direct launch under --no-graph)."""
     import matrixinversion_amd as mia
 
     batch = args.batch or 500_000
     info = mia.device_info(0)
     log(f"[ragged] device {info.arch} CUs={info.cu_count}")
-    d = ragged_measure(batch, args.pivot, args.steps, args.warmup, args.n_lo, args.n_hi)
-    ms = d["graph_ms_mean"]
+    d = ragged_measure(batch, args.pivot, args.steps, args.warmup, args.n_lo, args.n_hi, graph=not args.no_graph)
+    ms = d["step_ms_mean"]
     algo = d["algorithmic_bytes_per_launch"]
     achieved = algo / (ms * 1e-3) / 1e9
     traffic = load_traffic(f"traffic_lu_ragged_n{args.n_lo}-{args.n_hi}_{args.pivot}_{batch}.json")
@@ -356,7 +371,8 @@ def bench_ragged(args) -> int:
                 "U[-1,1), packed in matrix order",
         "config": {"workload": f"ragged in-place LU inversion, {batch} matrices, n~U{{{args.n_lo}..{args.n_hi}}}, "
                                f"pivot={args.pivot}, "
-                               f"int64 offsets, one hipGraph-replayed plan + inversion launch",
+                               f"int64 offsets, plan + inversion launch "
+                               f"({'launched directly' if args.no_graph else 'one hipGraph replay'})",
                    "batch": batch, "pivot": args.pivot, "global_batch": batch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic["bytes"],
@@ -673,6 +689,9 @@ def main() -> int:
     ap.add_argument("--rates", default="1000,4000,16000", help="serve: request rates (req/s)")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per rate")
     ap.add_argument("--no-serve", action="store_true", help="headline: skip the short serving run")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="ragged: launch directly instead of replaying hipGraphs (profiled runs: rocprofv3's "
+                         "kernel trace stops the process inside hipGraphLaunch on this image)")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline: skip the dominant-input gate and the mixed-n leg (profiling runs)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
