@@ -8,24 +8,29 @@
 // on one stream without host synchronisation or allocation (so the sequence
 // can be captured once in a hipGraph and replayed):
 //
-//   hist      matrices per n in each block of 1024 (LDS histograms), one
-//             row of counts per block, no global atomics
-//   scan      one workgroup: the class table -- size classes ordered largest
-//             n first; per class its first slot, its first descriptor, its
-//             lane layout and the matrices per descriptor (MPW(n) * grab(n)
-//             <= 64: MPW is 4 with 16-lane rows, 16 with quads) -- and each
-//             block's first slot in each class
-//   scatter   each matrix's offset and pivot offset into its class's slot
-//             range (its block's first slot + a rank from an LDS atomic), so
-//             a descriptor's matrices sit in consecutive slots
+//   hist      matrices per n in each block of 1024 (LDS histograms), written
+//             as the block's column of a class-major block table; no global
+//             atomics
+//   scan      one workgroup per class: the class's row of the block table
+//             becomes each block's first slot within the class, and the
+//             class total goes to the header
+//   scatter   every block derives the class table from the 32 totals (block
+//             0 publishes it) -- size classes ordered largest n first; per
+//             class its first slot, its first descriptor, its lane layout and
+//             the matrices per descriptor (MPW(n) * grab(n) <= 64: MPW is 4
+//             with 16-lane rows, 16 with quads) -- then writes each matrix's
+//             offset and pivot offset to its slot (class first + the block's
+//             first slot + a rank from an LDS atomic), so a descriptor's
+//             matrices sit in consecutive slots
 //   invert    one workgroup (one wave) per descriptor, classes largest first
 //             so the tail of the launch is made of the cheapest work.  A
 //             descriptor is located from the class table held in lanes 0..32
 //             (ballot + readlane; the 1.5 KB table is read by every wave and
 //             stays cached), its up to 64 slot offsets arrive with one vector
 //             load (one per lane), and its tiles run the plain kernel's tile
-//             routines with per-matrix gather loads and scatter stores; the
-//             loads of tile b+1 are issued before the stores of tile b.
+//             routines; each matrix is gathered and scattered by its own lane
+//             group, and the loads of tile b+1 are issued before the stores
+//             of tile b.
 //
 // A matrix's result does not depend on which tile or slot it lands in: every
 // tile routine treats its matrices independently (the serial fallback of the
