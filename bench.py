@@ -405,16 +405,18 @@ def load_traffic(name: str) -> dict:
     return {"bytes": None if stale else t.get("hbm_bytes_per_launch"), "build_id": t.get("build_id"), "stale": stale}
 
 
-def serve_sweep(rates, duration: float, budget_gb: float = 64.0, pool: int = 200_000) -> list:
-    """Poisson serving runs (matrixinversion_amd.serve) at each rate."""
+def serve_sweep(rates, duration: float, budget_gb: float = 64.0, pool: int = 200_000, driver: str = "native") -> list:
+    """Poisson serving runs (matrixinversion_amd.serve) at each rate, driven
+    by the library's host loop (native) or by the Python loop."""
     from matrixinversion_amd import serve
 
     pools = (serve.Pool(42, pool), serve.Pool(43, pool, dominant=True))
+    run = serve.poisson_stream_native if driver == "native" else serve.poisson_stream
     out = []
     with serve.Server(int(budget_gb * 2**30)) as srv:
-        serve.warm(srv, pools)  # untimed: first kernel launches load code objects
+        serve.warm(srv, pools, driver=driver)  # untimed: first kernel launches load code objects
         for r in rates:
-            res = serve.poisson_stream(srv, pools, float(r), duration)
+            res = run(srv, pools, float(r), duration)
             log(f"[serve] {r:>8.0f} req/s: p50 {res['p50_ms']:.3f} ms, p99 {res['p99_ms']:.3f} ms, "
                 f"{res['matrices_per_s'] / 1e6:.2f} M matrices/s, {res['matrices_per_batch']:.0f} matrices/batch")
             out.append(res)
@@ -429,7 +431,7 @@ def bench_serve(args) -> int:
     diagonally dominant, 64 GB HBM admission budget.  One JSON line; value =
     p99 latency at the reference rate (the last of --rates)."""
     rates = [float(x) for x in args.rates.split(",")]
-    sweep = serve_sweep(rates, args.duration)
+    sweep = serve_sweep(rates, args.duration, driver=args.serve_driver)
     top = sweep[-1]
     out = {
         "metric": "p50/p99 ms, mixed serving stream",
@@ -446,7 +448,7 @@ def bench_serve(args) -> int:
         "data": "synthetic: request pools from mt19937(42/43), Poisson arrivals from RandomState(7)",
         "config": {"workload": "serving: Poisson arrivals, log-uniform 1..4096 matrices per request, n~U{4..32}, "
                                "70% pivoted / 30% unpivoted dominant, 64 GB admission budget",
-                   "rates_req_s": rates, "duration_s": args.duration},
+                   "rates_req_s": rates, "duration_s": args.duration, "driver": args.serve_driver},
         "detail": {"p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "matrices_per_s": top["matrices_per_s"],
                    "sweep": sweep},
         "roofline": None,
@@ -689,6 +691,8 @@ def main() -> int:
     ap.add_argument("--rates", default="1000,4000,16000", help="serve: request rates (req/s)")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per rate")
     ap.add_argument("--no-serve", action="store_true", help="headline: skip the short serving run")
+    ap.add_argument("--serve-driver", choices=["native", "python"], default="native",
+                    help="serving runs: the library's host loop over lu_server_* (native) or the Python loop")
     ap.add_argument("--no-graph", action="store_true",
                     help="ragged: launch directly instead of replaying hipGraphs (profiled runs: rocprofv3's "
                          "kernel trace stops the process inside hipGraphLaunch on this image)")
@@ -828,11 +832,12 @@ def main() -> int:
             ok &= rg["ok"]
     if not args.no_serve:
         # the metric's second half: latency of the mixed serving stream
-        top = serve_sweep([16000.0], 2.0)[0]
+        top = serve_sweep([16000.0], 2.0, driver=args.serve_driver)[0]
         out["serving"] = {"rate_req_s": top["rate_req_s"], "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
                           "matrices_per_s": top["matrices_per_s"],
                           "workload": "Poisson arrivals, log-uniform 1..4096 matrices/request, n~U{4..32}, "
-                                      "70% pivoted / 30% unpivoted dominant, 64 GB admission budget"}
+                                      "70% pivoted / 30% unpivoted dominant, 64 GB admission budget",
+                          "driver": top["driver"], "busy_retries": top["busy_retries"]}
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(n, gepp=args.pivot == "gepp")

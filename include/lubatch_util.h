@@ -63,6 +63,18 @@ int   lubx_scatter_f32(float* packed, const float* src, const int64_t* offsets, 
  * FETCH_SIZE / WRITE_SIZE counters against a known byte count. */
 int   lubx_copy_f32(float* dst, const float* src, size_t count, void* stream);
 
+/* Benchmark driver for lu_server_*: replays a request schedule from a host
+ * loop with no interpreter in it.  Request i is due arrival_s[i] seconds
+ * after the call (ascending): count[i] matrices packed at address src[i],
+ * sizes ns[ns_off[i] ..], flags[i].  latency_ms[i] = arrival to the loop pass
+ * that saw it done; stats[0] = batches launched, stats[1] = LUB_EBUSY
+ * refusals.  LUB_EHIP if the stream has not drained 30 s after its last
+ * arrival. */
+struct lub_server;
+int   lubx_serve_replay(struct lub_server* server, int n_req, const double* arrival_s, const uint64_t* src,
+                        const int* ns, const int64_t* ns_off, const int* count, const int* flags,
+                        double* latency_ms, int64_t* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,7 @@ UTIL = {
     "lubx_event_sync": (_i, [_vp]),
     "lubx_event_elapsed_ms": (ctypes.c_float, [_vp, _vp]),
     "lubx_copy_f32": (_i, [_vp, _vp, _sz, _vp]),
+    "lubx_serve_replay": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lubx_gather_f32": (_i, [_vp, _vp, _vp, _i, _i64, _vp]),
     "lubx_scatter_f32": (_i, [_vp, _vp, _vp, _i, _i64, _vp]),
     "lubx_graph_capture_begin": (_i, [_vp]),

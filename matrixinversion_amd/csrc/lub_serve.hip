@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "lubatch.h"
+#include "lubatch_util.h"
 
 namespace {
 
@@ -446,6 +447,70 @@ int lu_server_counters(lub_server* s, int64_t* batches, int* peak_rule_inflight)
   if (s == nullptr) return LUB_ENULL;
   if (batches) *batches = s->batches;
   if (peak_rule_inflight) *peak_rule_inflight = s->peak_rule_inflight;
+  return LUB_OK;
+}
+
+
+// Benchmark driver: replays a request schedule against `server` from a host
+// loop with no interpreter in it (declared in lubatch_util.h).  Request i
+// becomes due arrival_s[i] seconds after the call starts (ascending); it is
+// count[i] matrices packed at src[i] (an address) with sizes
+// ns[ns_off[i] .. ns_off[i] + count[i]) and flags[i].  The loop submits every
+// due request (LUB_EBUSY: the rest wait for the next pass), calls progress,
+// and on completions polls and releases; latency_ms[i] is the time from
+// arrival_s[i] to the pass that saw request i done.  stats[0] = batches
+// launched, stats[1] = LUB_EBUSY refusals.  Gives up with LUB_EHIP if the
+// stream has not drained 30 s after its last arrival.
+int lubx_serve_replay(lub_server* s, int n_req, const double* arrival_s, const uint64_t* src, const int* ns,
+                      const int64_t* ns_off, const int* count, const int* flags, double* latency_ms, int64_t* stats) {
+  if (s == nullptr || arrival_s == nullptr || src == nullptr || ns == nullptr || ns_off == nullptr ||
+      count == nullptr || flags == nullptr || latency_ms == nullptr || stats == nullptr)
+    return LUB_ENULL;
+  if (n_req < 0) return LUB_EINVAL_ARG;
+  std::vector<std::pair<uint64_t, int>> live;  // (ticket, request), few at a time
+  std::vector<uint64_t> polled(256);
+  const double give_up = (n_req > 0 ? arrival_s[n_req - 1] : 0.0) + 30.0;
+  const Clock::time_point t0 = Clock::now();
+  auto elapsed = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+  int nxt = 0, done = 0;
+  int64_t batches = 0, busy = 0;
+  while (done < n_req) {
+    const double now = elapsed();
+    if (now > give_up) return LUB_EHIP;
+    while (nxt < n_req && arrival_s[nxt] <= now) {
+      uint64_t t = 0;
+      const int rc = lu_server_submit(s, reinterpret_cast<const float*>(src[nxt]), ns + ns_off[nxt], count[nxt],
+                                      flags[nxt], &t);
+      if (rc == LUB_EBUSY) {
+        ++busy;
+        break;
+      }
+      if (rc != LUB_OK) return rc;
+      live.emplace_back(t, nxt++);
+    }
+    int launched = 0, completed = 0;
+    if (int rc = lu_server_progress(s, &launched, &completed); rc != LUB_OK) return rc;
+    batches += launched;
+    if (completed > 0) {
+      const double seen = elapsed();
+      for (;;) {
+        int got = 0;
+        if (int rc = lu_server_poll(s, polled.data(), static_cast<int>(polled.size()), &got); rc != LUB_OK) return rc;
+        for (int k = 0; k < got; ++k) {
+          auto it = std::find_if(live.begin(), live.end(), [&](const auto& e) { return e.first == polled[k]; });
+          if (it == live.end()) return LUB_EINVAL_ARG;  // not a ticket of this replay
+          latency_ms[it->second] = (seen - arrival_s[it->second]) * 1e3;
+          if (int rc = lu_server_release(s, it->first); rc != LUB_OK) return rc;
+          *it = live.back();
+          live.pop_back();
+          ++done;
+        }
+        if (got < static_cast<int>(polled.size())) break;
+      }
+    }
+  }
+  stats[0] = batches;
+  stats[1] = busy;
   return LUB_OK;
 }
 

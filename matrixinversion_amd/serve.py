@@ -156,15 +156,68 @@ def request_mix(seed: int, count: int, pool_size: int, max_count: int = 4096, pi
     return counts, starts, pivot
 
 
-def poisson_stream(server: Server, pools, rate: float, duration: float, seed: int = 7,
-                   max_count: int = 4096) -> dict:
-    """Drive `server` with Poisson arrivals; returns latency percentiles and
-    throughput.  pools = (pivoted pool, no-pivot pool)."""
+def schedule(pools, rate: float, duration: float, seed: int = 7, max_count: int = 4096):
+    """The request stream both drivers replay: (arrival times in s, matrix
+    counts, pool starts, pivoted?, flags)."""
     rng = np.random.RandomState(seed)
     n_req = max(1, int(rng.poisson(rate * duration)))
     arrivals = np.sort(rng.uniform(0.0, duration, n_req))
     counts, starts, pivot = request_mix(seed + 1, n_req, min(p.ns.size for p in pools), max_count)
     flags = np.where(pivot, api.PIVOT_PARALLEL, api.PIVOT_NONE)
+    return arrivals, counts, starts, pivot, flags
+
+
+def _summary(rate, lat, counts, pivot, wall, batches, busy_retries, peak, driver) -> dict:
+    mats = int(counts.sum())
+    return {"rate_req_s": rate, "requests": int(lat.size), "matrices": mats, "wall_s": wall,
+            "matrices_per_s": mats / wall, "requests_per_s": lat.size / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "batches": batches,
+            "matrices_per_batch": mats / max(batches, 1), "busy_retries": busy_retries,
+            "pivot_share": float(pivot.mean()), "peak_rule_inflight_since_create": peak, "driver": driver}
+
+
+def poisson_stream_native(server: Server, pools, rate: float, duration: float, seed: int = 7,
+                          max_count: int = 4096) -> dict:
+    """The same stream as poisson_stream, replayed by the library's host loop
+    (lubx_serve_replay): submit, progress, poll and release with no
+    interpreter between them, so the latencies are the engine's."""
+    arrivals, counts, starts, pivot, flags = schedule(pools, rate, duration, seed, max_count)
+    n_req = arrivals.size
+    src = np.empty(n_req, np.uint64)
+    ns_off = np.empty(n_req, np.int64)
+    ns_parts = []
+    off = 0
+    for i in range(n_req):
+        pool = pools[0] if pivot[i] else pools[1]
+        ptr, ns = pool.request(int(starts[i]), int(counts[i]))
+        src[i] = ptr
+        ns_off[i] = off
+        ns_parts.append(ns)
+        off += ns.size
+    ns_all = np.ascontiguousarray(np.concatenate(ns_parts), dtype=np.int32)
+    cnt = np.ascontiguousarray(counts, dtype=np.int32)
+    flg = np.ascontiguousarray(flags, dtype=np.int32)
+    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    lat = np.full(n_req, np.nan)
+    stats = np.zeros(2, np.int64)
+    _, peak0 = server.counters()
+    t0 = time.perf_counter()
+    check(server._lib.lubx_serve_replay(server.handle, int(n_req), arr.ctypes.data, src.ctypes.data,
+                                        ns_all.ctypes.data, ns_off.ctypes.data, cnt.ctypes.data, flg.ctypes.data,
+                                        lat.ctypes.data, stats.ctypes.data), "lubx_serve_replay")
+    wall = time.perf_counter() - t0
+    _, peak = server.counters()
+    return _summary(rate, lat, counts, pivot, wall, int(stats[0]), int(stats[1]), peak, "native")
+
+
+def poisson_stream(server: Server, pools, rate: float, duration: float, seed: int = 7,
+                   max_count: int = 4096) -> dict:
+    """Drive `server` with Poisson arrivals from a Python loop; returns
+    latency percentiles and throughput.  pools = (pivoted pool, no-pivot
+    pool)."""
+    arrivals, counts, starts, pivot, flags = schedule(pools, rate, duration, seed, max_count)
+    n_req = arrivals.size
     lat = np.full(n_req, np.nan)
     busy_retries = 0
     inflight = {}          # ticket -> request index
@@ -195,20 +248,14 @@ def poisson_stream(server: Server, pools, rate: float, duration: float, seed: in
                 done += 1
     wall = time.perf_counter() - t0
     b1, peak = server.counters()
-    mats = int(counts.sum())
-    return {"rate_req_s": rate, "requests": n_req, "matrices": mats, "wall_s": wall,
-            "matrices_per_s": mats / wall, "requests_per_s": n_req / wall,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "batches": batches,
-            "matrices_per_batch": mats / max(batches, 1), "busy_retries": busy_retries,
-            "pivot_share": float(pivot.mean()), "peak_rule_inflight_since_create": peak}
+    return _summary(rate, lat, counts, pivot, wall, batches, busy_retries, peak, "python")
 
 
-def warm(server: Server, pools, rate: float = 4000.0, duration: float = 0.15) -> None:
+def warm(server: Server, pools, rate: float = 4000.0, duration: float = 0.15, driver: str = "python") -> None:
     """An untimed stream through `server` (both pivot rules): the first launch
     of each kernel loads its code object, which a measured stream should not
     see."""
-    poisson_stream(server, pools, rate, duration, seed=1001)
+    (poisson_stream_native if driver == "native" else poisson_stream)(server, pools, rate, duration, seed=1001)
 
 
 def main(argv=None) -> int:
@@ -217,15 +264,18 @@ def main(argv=None) -> int:
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--pool", type=int, default=200_000)
+    ap.add_argument("--driver", choices=["native", "python"], default="native",
+                    help="native: the library's host loop (lubx_serve_replay); python: this module's loop")
     args = ap.parse_args(argv)
     info = device_info(0)
     print(f"[serve] {info.name} {info.arch}", file=sys.stderr, flush=True)
     pools = (Pool(42, args.pool), Pool(43, args.pool, dominant=True))
     out = []
+    run = poisson_stream_native if args.driver == "native" else poisson_stream
     with Server(int(args.budget_gb * 2**30)) as srv:
-        warm(srv, pools)
+        warm(srv, pools, driver=args.driver)
         for r in (float(x) for x in args.rates.split(",")):
-            res = poisson_stream(srv, pools, r, args.duration)
+            res = run(srv, pools, r, args.duration)
             print(json.dumps(res), flush=True)
             out.append(res)
     for p in pools:

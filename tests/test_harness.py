@@ -102,3 +102,26 @@ def test_host_checks_count_mismatches_and_errors():
     x2 = x.copy()
     x2[7] *= 1.01
     assert harness.host_checks(a, x2, piv, api.PIVOT_PARALLEL)[0] > 5e-3
+
+
+def test_serving_drivers_share_one_schedule():
+    """The native and the Python serving drivers replay the same stream:
+    schedule() is what both build their requests from, and it is a pure
+    function of its seed."""
+    import numpy as np
+
+    from matrixinversion_amd import serve
+
+    class FakePool:
+        def __init__(self, size):
+            self.ns = np.full(size, 8, np.int32)
+
+    pools = (FakePool(5000), FakePool(5000))
+    a = serve.schedule(pools, 3000.0, 0.2, seed=11)
+    b = serve.schedule(pools, 3000.0, 0.2, seed=11)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    arrivals, counts, starts, pivot, flags = a
+    assert np.all(np.diff(arrivals) >= 0) and arrivals.max() <= 0.2
+    assert counts.min() >= 1 and counts.max() <= 4096 and np.all(starts + counts <= 5000)
+    assert set(np.unique(flags)) <= {1, 0}
