@@ -184,6 +184,46 @@ __global__ __launch_bounds__(kPlanThreads) void ragged_scatter(const int* __rest
   }
 }
 
+// The three planning steps in one workgroup, for batches of up to
+// kSmallPlanMax matrices (serving batches): an LDS histogram, the class
+// table (published), then each matrix's slot from an LDS-atomic rank within
+// its class.  One launch instead of three; the header and slot arrays it
+// writes have the same meaning (slot order within a class is not fixed, and
+// results do not depend on it).
+constexpr int kSmallPlanThreads = 1024;
+constexpr int64_t kSmallPlanMax = 16384;
+
+__global__ __launch_bounds__(kSmallPlanThreads) void ragged_plan_small(const int* __restrict__ ns,
+                                                                       const int64_t* __restrict__ offs,
+                                                                       const int64_t* __restrict__ poffs, int64_t batch,
+                                                                       int* __restrict__ ws, RaggedTuning tune,
+                                                                       int64_t* __restrict__ soffs,
+                                                                       int64_t* __restrict__ spoffs) {
+  __shared__ int hist[RaggedWs::BSTRIDE];
+  __shared__ int first_slot[33];
+  __shared__ int cur[33];
+  const int t = threadIdx.x;
+  if (t < RaggedWs::BSTRIDE) hist[t] = 0;
+  __syncthreads();
+  for (int64_t i = t; i < batch; i += kSmallPlanThreads) {
+    const int n = ns[i];
+    atomicAdd(&hist[(n >= 1 && n <= 32) ? n : 0], 1);
+  }
+  __syncthreads();
+  if (t >= 1 && t <= 32) ws[RaggedWs::COUNTS + t] = hist[t];  // read back below by the same thread
+  ragged_class_table(ws, batch, tune, true, first_slot);
+  if (t >= 1 && t <= 32) cur[t] = first_slot[t];
+  __syncthreads();
+  for (int64_t i = t; i < batch; i += kSmallPlanThreads) {
+    const int n = ns[i];
+    if (n >= 1 && n <= 32) {
+      const int pos = atomicAdd(&cur[n], 1);
+      soffs[pos] = offs[i];
+      if (poffs != nullptr) spoffs[pos] = poffs[i];
+    }
+  }
+}
+
 }  // namespace lub
 
 namespace {
@@ -269,12 +309,17 @@ int enqueue_ragged(char* ws, int64_t capacity, int cus, float* A, const int64_t*
   int* bhist = reinterpret_cast<int*>(ws + lub::RaggedWs::bhist_offset());
   int64_t* soffs = reinterpret_cast<int64_t*>(ws + lub::RaggedWs::slots_offset(capacity));
   int64_t* spoffs_all = soffs + capacity;
-  const unsigned blocks = plan_blocks(batch);
-  const int64_t nblocks = blocks;
-  lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, bhist, nblocks);
-  lub::ragged_scan<<<32, lub::kPlanThreads, 0, s>>>(hdr, bhist, nblocks);
-  lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, hdr,
-                                                            bhist, nblocks, rt, soffs, spoffs_all);
+  if (batch <= lub::kSmallPlanMax) {
+    lub::ragged_plan_small<<<1, lub::kSmallPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, hdr,
+                                                                rt, soffs, spoffs_all);
+  } else {
+    const unsigned blocks = plan_blocks(batch);
+    const int64_t nblocks = blocks;
+    lub::ragged_hist<<<blocks, lub::kPlanThreads, 0, s>>>(ns, batch, bhist, nblocks);
+    lub::ragged_scan<<<32, lub::kPlanThreads, 0, s>>>(hdr, bhist, nblocks);
+    lub::ragged_scatter<<<blocks, lub::kPlanThreads, 0, s>>>(ns, offsets, piv ? piv_offsets : nullptr, batch, hdr,
+                                                              bhist, nblocks, rt, soffs, spoffs_all);
+  }
   int64_t* spoffs = piv ? spoffs_all : nullptr;
   const int* hdr_arg = hdr;
   void* args[] = {&A, &soffs, &piv, &spoffs, &hdr_arg};
