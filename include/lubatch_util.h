@@ -68,7 +68,9 @@ int   lubx_copy_f32(float* dst, const float* src, size_t count, void* stream);
  * after the call (ascending): count[i] matrices packed at address src[i],
  * sizes ns[ns_off[i] ..], flags[i].  latency_ms[i] = arrival to the loop pass
  * that saw it done; stats[0] = batches launched, stats[1] = LUB_EBUSY
- * refusals.  LUB_EHIP if the stream has not drained 30 s after its last
+ * refusals.  Requests submitted before the call are not released; their
+ * completions are taken off the poll queue (lu_server_query still reports
+ * them).  LUB_EHIP if the stream has not drained 30 s after its last
  * arrival. */
 struct lub_server;
 int   lubx_serve_replay(struct lub_server* server, int n_req, const double* arrival_s, const uint64_t* src,

@@ -458,7 +458,9 @@ int lu_server_counters(lub_server* s, int64_t* batches, int* peak_rule_inflight)
 // ns[ns_off[i] .. ns_off[i] + count[i]) and flags[i].  The loop submits every
 // due request (LUB_EBUSY: the rest wait for the next pass), calls progress,
 // and on completions polls and releases; latency_ms[i] is the time from
-// arrival_s[i] to the pass that saw request i done.  stats[0] = batches
+// arrival_s[i] to the pass that saw request i done.  Tickets of requests
+// submitted before the call are taken off the poll queue but not released
+// (lu_server_query still reports them).  stats[0] = batches
 // launched, stats[1] = LUB_EBUSY refusals.  Gives up with LUB_EHIP if the
 // stream has not drained 30 s after its last arrival.
 int lubx_serve_replay(lub_server* s, int n_req, const double* arrival_s, const uint64_t* src, const int* ns,
@@ -498,7 +500,7 @@ int lubx_serve_replay(lub_server* s, int n_req, const double* arrival_s, const u
         if (int rc = lu_server_poll(s, polled.data(), static_cast<int>(polled.size()), &got); rc != LUB_OK) return rc;
         for (int k = 0; k < got; ++k) {
           auto it = std::find_if(live.begin(), live.end(), [&](const auto& e) { return e.first == polled[k]; });
-          if (it == live.end()) return LUB_EINVAL_ARG;  // not a ticket of this replay
+          if (it == live.end()) continue;  // a ticket the caller submitted before the replay: left to the caller
           latency_ms[it->second] = (seen - arrival_s[it->second]) * 1e3;
           if (int rc = lu_server_release(s, it->first); rc != LUB_OK) return rc;
           *it = live.back();
