@@ -142,7 +142,7 @@ def cpu_baseline_ragged(ns_all: np.ndarray, flat: np.ndarray, offsets: np.ndarra
 
 
 def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 2, n_hi: int = 32,
-                   graph: bool = True) -> dict:
+                   graph: bool = True, order: str = "random") -> dict:
     """Mixed-n batch (`batch` matrices, n ~ U{n_lo..n_hi}, packed with int64
     offsets): the ragged entry (plan + inversion kernel) eager and replayed
     from a hipGraph, against one plain launch per n (a) over the same matrices
@@ -150,14 +150,24 @@ def ragged_measure(batch: int, pivot: str, steps: int, warmup: int, n_lo: int = 
     gather cost counted) and (b) on the packed batch itself (gather each class,
     invert, scatter back).  graph=False launches everything directly (no
     hipGraph), for profilers that do not follow graph launches; the step time
-    is then the eager one.  Returns the measurements; `ok` is False when any
-    result differs bitwise."""
+    is then the eager one.  order="sorted" packs the same matrices grouped by
+    n (largest first) instead of in generation order.  Returns the
+    measurements; `ok` is False when any result differs bitwise."""
     import matrixinversion_amd as mia
     from matrixinversion_amd import workloads
 
     flags = mia.PIVOT_NAMES[pivot]
     t0 = time.perf_counter()
     flat, offsets, ns, poffs = workloads.ragged_batch(42, batch, n_lo, n_hi)
+    if order == "sorted":  # the same matrices, packed class by class
+        perm = np.argsort(-ns, kind="stable")
+        flat = np.concatenate([flat[offsets[i]:offsets[i] + int(ns[i]) ** 2] for i in perm])
+        ns = ns[perm]
+        sq = ns.astype(np.int64) ** 2
+        offsets = np.zeros(batch, np.int64)
+        np.cumsum(sq[:-1], out=offsets[1:])
+        poffs = np.zeros(batch, np.int64)
+        np.cumsum(ns[:-1].astype(np.int64), out=poffs[1:])
     log(f"[ragged] generated {batch} matrices ({flat.nbytes / 1e6:.0f} MB) in {time.perf_counter() - t0:.1f}s; "
         f"E[n]={ns.mean():.2f} E[n^2]={(ns.astype(np.int64) ** 2).mean():.1f}")
     total_piv = int(ns.sum())
@@ -350,11 +360,13 @@ def bench_ragged(args) -> int:
     batch = args.batch or 500_000
     info = mia.device_info(0)
     log(f"[ragged] device {info.arch} CUs={info.cu_count}")
-    d = ragged_measure(batch, args.pivot, args.steps, args.warmup, args.n_lo, args.n_hi, graph=not args.no_graph)
+    d = ragged_measure(batch, args.pivot, args.steps, args.warmup, args.n_lo, args.n_hi, graph=not args.no_graph,
+                       order=args.ragged_order)
     ms = d["step_ms_mean"]
     algo = d["algorithmic_bytes_per_launch"]
     achieved = algo / (ms * 1e-3) / 1e9
-    traffic = load_traffic(f"traffic_lu_ragged_n{args.n_lo}-{args.n_hi}_{args.pivot}_{batch}.json")
+    traffic = (load_traffic(f"traffic_lu_ragged_n{args.n_lo}-{args.n_hi}_{args.pivot}_{batch}.json")
+               if args.ragged_order == "random" else {"bytes": None, "build_id": None, "stale": None})
     out = {
         "metric": "matrices/s (ragged mixed-n batch, pivoted)",
         "value": batch / (ms * 1e-3),
@@ -368,12 +380,12 @@ def bench_ragged(args) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": f"synthetic: sizes mt19937(43) n={args.n_lo}+x%{args.n_hi - args.n_lo + 1}, entries mt19937(42) "
-                "U[-1,1), packed in matrix order",
+                f"U[-1,1), packed {'in matrix order' if args.ragged_order == 'random' else 'grouped by n'}",
         "config": {"workload": f"ragged in-place LU inversion, {batch} matrices, n~U{{{args.n_lo}..{args.n_hi}}}, "
                                f"pivot={args.pivot}, "
                                f"int64 offsets, plan + inversion launch "
                                f"({'launched directly' if args.no_graph else 'one hipGraph replay'})",
-                   "batch": batch, "pivot": args.pivot, "global_batch": batch},
+                   "batch": batch, "pivot": args.pivot, "global_batch": batch, "order": args.ragged_order},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic["bytes"],
                      "traffic_build_id": traffic["build_id"], "traffic_stale": traffic["stale"]},
@@ -684,6 +696,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n-lo", type=int, default=2, help="ragged: smallest n")
     ap.add_argument("--n-hi", type=int, default=32, help="ragged: largest n")
+    ap.add_argument("--ragged-order", choices=["random", "sorted"], default="random",
+                    help="ragged: matrices packed in generation order (random n) or grouped by n")
     ap.add_argument("--workload", default="headline", choices=["headline", "ragged", "ragged_classes", "serve", "refine", "rocsolver"],
                     help="headline: the metric's configuration; ragged: mixed-n batch (--batch defaults to 500k); "
                          "serve: Poisson serving sweep; refine: precision workload (100k n=32); "
