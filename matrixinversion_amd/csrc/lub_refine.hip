@@ -170,8 +170,9 @@ int lu_batched_inverse_refined(const float* A, float* X, int* piv, int n, int64_
     const void* fn = lub::kernel_table().refined[n][rule == LUB_PIVOT_NONE ? 0 : 1];
     if (fn == nullptr) return LUB_ENODEVICE;
     int64_t ntiles = (batch + 3) / 4;
+    // one workgroup per tile (several tiles per workgroup measured slower)
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ntiles, int64_t(1) << 22));
-    int vec = ((reinterpret_cast<uintptr_t>(A) & 15u) == 0) ? 1 : 0;
+    int vec = ((reinterpret_cast<uintptr_t>(A) & 15u) == 0 ? 1 : 0) | ((reinterpret_cast<uintptr_t>(X) & 15u) == 0 ? 2 : 0);
     void* args[] = {const_cast<float**>(&A), &X, &piv, &batch, &ntiles, &vec};
     if (hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, s) != hipSuccess) return LUB_EHIP;
     return LUB_OK;

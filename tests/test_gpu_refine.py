@@ -49,20 +49,22 @@ def test_refinement_reduces_error(gpu, k):
         assert np.median(e1) <= np.median(e0)
 
 
-def _inverse_refined(gpu, a, flags, offset=0):
+def _inverse_refined(gpu, a, flags, offset=0, x_offset=0):
     """lu_batched_inverse_refined through the C ABI (A at `offset` bytes into
-    its buffer): (refined inverse, pivots, A after the call)."""
+    its buffer, X at `x_offset` bytes into its own): (refined inverse,
+    pivots, A after the call)."""
     from matrixinversion_amd._native import check, lib
 
     batch, n, _ = a.shape
     A = gpu.DeviceBuffer(a.nbytes + offset)
-    X = gpu.DeviceBuffer(a.nbytes)
+    X = gpu.DeviceBuffer(a.nbytes + x_offset)
     P = gpu.DeviceBuffer(batch * n * 4)
     try:
         A.upload(a, offset=offset)
-        check(lib().lu_batched_inverse_refined(A.ptr + offset, X.ptr, P.ptr, n, batch, flags, None), "refined")
+        check(lib().lu_batched_inverse_refined(A.ptr + offset, X.ptr + x_offset, P.ptr, n, batch, flags, None),
+              "refined")
         gpu.synchronize()
-        return (X.download(a.shape, np.float32), P.download((batch, n), np.int32),
+        return (X.download(a.shape, np.float32, offset=x_offset), P.download((batch, n), np.int32),
                 A.download(a.shape, np.float32, offset=offset))
     finally:
         for b in (A, X, P):
@@ -93,15 +95,17 @@ def test_inverse_refined_entry_keeps_a(gpu):
 def test_fused_refinement_every_size(gpu, flags):
     """n = 17..32 take the fused kernel (inverse + refinement with A read
     once): its result is one refinement step of the plain inverse, within
-    4 ulp, for batches ending in a partial tile and for an unaligned A (the
-    element-wise tile path); pivots equal the oracle's; A is untouched."""
+    4 ulp, for batches ending in a partial tile, for an unaligned A (the
+    element-wise tile load) and for an unaligned X (the element-wise tile
+    store); pivots equal the oracle's; A is untouched."""
     rule = oracle.PIV_NONE if flags == 0 else oracle.PIV_FIRSTMAX
     for n in range(17, 33):
-        for batch, offset in ((23, 0), (9, 4)):
+        for batch, offset, x_offset in ((23, 0, 0), (9, 4, 0), (12, 0, 4)):
             a = oracle.dominant_matrices(700 + n, batch, n) if flags == 0 else \
                 workloads.conditioned_matrices(800 + n, batch, n, 2)
-            print(f"[case] fused refine n={n} flags={flags:#x} batch={batch} offset={offset}", flush=True)
-            x, p, a_after = _inverse_refined(gpu, a, flags, offset)
+            print(f"[case] fused refine n={n} flags={flags:#x} batch={batch} offset={offset} x_offset={x_offset}",
+                  flush=True)
+            x, p, a_after = _inverse_refined(gpu, a, flags, offset, x_offset)
             assert np.array_equal(a_after, a), f"n={n}: A was modified"
             x0, _ = gpu.invert(a, flags)
             assert _within_step(a, x0, x), f"n={n}: not one refinement step of the plain inverse"
