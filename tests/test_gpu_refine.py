@@ -110,3 +110,47 @@ def test_fused_refinement_every_size(gpu, flags):
             x0, _ = gpu.invert(a, flags)
             assert _within_step(a, x0, x), f"n={n}: not one refinement step of the plain inverse"
             assert (p == oracle.batch_pivots(a, rule)).all(), f"n={n}: pivots"
+
+
+def test_fused_refinement_more_tiles_than_the_grid(gpu):
+    """2^22 tiles of four n = 32 matrices plus 1,000 tiles and 3 matrices
+    (16.8M matrices; A and X 68.7 GB each): the grid is capped at 2^22
+    workgroups, so the first 1,001 stride to a second tile and fetch it into
+    registers during their first tile's refinement, and the batch ends in a
+    partial tile.  The batch repeats a block of 16,384 matrices; windows at
+    the start, across the grid wrap and at the end equal the fused kernel's
+    result on the block itself, bit for bit, and A is untouched there."""
+    from matrixinversion_amd._native import check, lib
+
+    n, block_count = 32, 16_384
+    batch = (1 << 22) * 4 + 4 * 1000 + 3
+    block = workloads.conditioned_matrices(820, block_count, n, 2)
+    xb, pb, _ = _inverse_refined(gpu, block, 0x1)
+    per = n * n * 4
+    print(f"[case] fused refine n={n} batch={batch} ({2 * batch * per / 1e9:.1f} GB)", flush=True)
+    A = gpu.DeviceBuffer(batch * per)
+    X = gpu.DeviceBuffer(batch * per)
+    P = gpu.DeviceBuffer(batch * n * 4)
+    src = gpu.DeviceBuffer.from_array(block)
+    s = gpu.Stream()
+    try:
+        done = 0
+        while done < batch:
+            k = min(block_count, batch - done)
+            check(lib().lubx_memcpy_dtod_async(A.ptr + done * per, src.ptr, k * per, s.handle), "d2d")
+            done += k
+        check(lib().lu_batched_inverse_refined(A.ptr, X.ptr, P.ptr, n, batch, 0x1, s.handle), "refined")
+        s.synchronize()
+        wrap = (1 << 22) * 4  # first matrix of tile 2^22: the first strided iteration
+        for start in (0, wrap - 2048, batch - 4096):
+            x = X.download((4096, n, n), np.float32, offset=start * per)
+            p = P.download((4096, n), np.int32, offset=start * n * 4)
+            a = A.download((4096, n, n), np.float32, offset=start * per)
+            idx = (start + np.arange(4096)) % block_count
+            assert np.array_equal(x.view(np.uint32), xb[idx].view(np.uint32)), f"window at {start}"
+            assert (p == pb[idx]).all(), f"pivots at {start}"
+            assert np.array_equal(a, block[idx]), f"A changed at {start}"
+    finally:
+        s.close()
+        for b in (A, X, P, src):
+            b.close()
