@@ -426,7 +426,9 @@ def serve_sweep(rates, duration: float, budget_gb: float = 64.0, pool: int = 200
     run = serve.poisson_stream_native if driver == "native" else serve.poisson_stream
     out = []
     with serve.Server(int(budget_gb * 2**30)) as srv:
-        serve.warm(srv, pools, driver=driver)  # untimed: first kernel launches load code objects
+        # untimed: the first kernel launches load code objects, and the stream
+        # settles only after some time at the timed rate
+        serve.warm(srv, pools, rate=float(rates[0]), duration=0.5, driver=driver)
         for r in rates:
             res = run(srv, pools, float(r), duration)
             log(f"[serve] {r:>8.0f} req/s: p50 {res['p50_ms']:.3f} ms, p99 {res['p99_ms']:.3f} ms, "
