@@ -855,7 +855,7 @@ def main() -> int:
         if n == 32 and args.pivot in ("parallel", "serial", "tree"):
             # the mixed-size target: the ragged path against per-n plain launches
             log("[bench] mixed-n leg (500k matrices, n~U{2..32}) ...")
-            rg = ragged_measure(500_000, args.pivot, 5, 3)
+            rg = ragged_measure(500_000, args.pivot, 5, 20)  # 20 settle launches, as --workload ragged
             out["detail"]["mixed_n_500k"] = rg
             ok &= rg["ok"]
     if serving is not None:
